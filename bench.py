@@ -1,0 +1,275 @@
+"""Benchmark of the magnetic hot path on one MI355X.
+
+One step = plan (re)build + magnetization solve of (I - kappa G) x = H_ext to
+1e-6 relative residual (CG) + force F = (B(m) + Text) m, with the particle
+positions already resident in HBM.  The default workload is the 1,153,433
+particle pool (256x256x16 jittered lattice + 104,857 random particles above
+it, h = 2, chi = 1, H = (0,0,1)).
+
+Printed JSON line (rank 0): metric/value/unit per the contract, plus
+  "error"       : ||F - F*|| / ||F*|| on a seeded subsample, F* the FP64
+                  all-pairs force at the FP64 solution (solved to 1e-12),
+  "allpairs"    : Gpair-interactions/s of the FP32 all-pairs field kernel on
+                  the 262,144-particle cylinder,
+  "roofline"    : dominant kernel, algorithmic FLOPs per launch / measured
+                  launch time against the FP32 peak,
+  "cpu_baseline": the CPU oracle timed on a bounded sample of the workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "magnetic-force_amd"))
+
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
+FIELD_FLOPS_PER_PAIR = 32       # far-branch field pair, FMA = 2, rsqrt = 1
+FORCE_FLOPS_PER_PAIR = 64       # far-branch 6-component force tensor pair
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def to_dev(a, dtype=None):
+    import torch
+    return torch.tensor(np.ascontiguousarray(np.asarray(a).T), dtype=dtype or torch.float32,
+                        device="cuda").contiguous()
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=3)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--workload", default="pool1m")
+    p.add_argument("--mode", default="auto", choices=["auto", "fmm", "allpairs"])
+    p.add_argument("--rtol", type=float, default=1e-6)
+    p.add_argument("--no-check", action="store_true")
+    p.add_argument("--check-samples", type=int, default=4096)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=15.0)
+    p.add_argument("--no-allpairs", action="store_true")
+    p.add_argument("--profile-only", action="store_true",
+                   help="only the timed steps (for rocprofv3 runs)")
+    return p.parse_args()
+
+
+class Stepper:
+    """Runs plan build + solve + force for one particle system."""
+
+    def __init__(self, ctx, system, mode, rtol):
+        import torch
+        import magforce
+        self.ctx = ctx
+        self.s = system
+        self.mode = mode
+        self.rtol = rtol
+        self.pos = to_dev(system.pos)
+        self.b = magforce.uniform_field(system.n, system.hconst)
+        self.x = torch.zeros_like(self.b)
+        self.F = torch.empty_like(self.b)
+        self.plan = None
+        self.stats = None
+
+    def step(self):
+        import magforce
+        if self.plan is None:
+            self.plan = magforce.Plan(self.ctx, self.pos, self.s.h, mode=self.mode)
+        else:
+            self.plan.update_positions(self.pos)
+        self.x.zero_()
+        res = self.plan.solve(self.s.kappa, self.b, x0=None, rtol=self.rtol)
+        self.x = res.x
+        self.stats = res.stats
+        self.plan.force(self.s.kappa * self.x, out=self.F)
+
+
+def measure_allpairs(ctx, log_prefix="allpairs"):
+    """Gpairs/s of the FP32 all-pairs field kernel on the 262,144 cylinder."""
+    import torch
+    import magforce
+    from magforce import configs
+    s = configs.cylinder262k()
+    pos = to_dev(s.pos)
+    rng = np.random.default_rng(0)
+    m = to_dev(rng.normal(size=(s.n, 3)) * 0.5)
+    plan = magforce.Plan(ctx, pos, s.h)
+    out = torch.empty_like(m)
+    plan.field(m, out=out)
+    torch.cuda.synchronize()
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plan.field(m, out=out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, cnt = ctx.kernel_time("field")
+    ctx.enable_timing(False)
+    pairs = float(s.n) * s.n
+    avg_ms = ms / max(cnt, 1)
+    res = {"workload": "cylinder262k", "n": s.n, "gpairs_per_s": pairs / (avg_ms * 1e-3) / 1e9,
+           "kernel_ms": avg_ms, "wall_ms_per_matvec": wall / reps * 1e3,
+           "tflops": pairs * FIELD_FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12}
+    res["frac_fp32_peak"] = res["tflops"] / FP32_PEAK_TFLOPS
+    log(f"{log_prefix}: {res}")
+    plan.close()
+    return res
+
+
+def check_error(ctx, st: Stepper, samples: int):
+    """||F - F*|| / ||F*|| over a seeded subsample; F* from the FP64 path."""
+    import torch
+    import magforce
+    s = st.s
+    t0 = time.perf_counter()
+    ap = magforce.Plan(ctx, st.pos, s.h, mode=magforce.MODE_ALLPAIRS)
+    b64 = st.b.double()
+    ref = ap.solve_f64(s.kappa, b64, rtol=1e-12, max_refine=6)
+    x64 = ref.x
+    rng = np.random.default_rng(12345)
+    k = min(samples, s.n)
+    idx = torch.tensor(np.sort(rng.choice(s.n, size=k, replace=False)), dtype=torch.int64,
+                       device="cuda")
+    pos64 = st.pos.double()
+    F_ref = magforce.force_subset_f64(ctx, pos64, s.kappa * x64, s.h, idx)
+    F = st.F[:, idx].double()
+    x_err = float(torch.linalg.norm(st.x.double() - x64) / torch.linalg.norm(x64))
+    err = float(torch.linalg.norm(F - F_ref) / torch.linalg.norm(F_ref))
+    rms = torch.sqrt((F_ref ** 2).sum(0).mean())
+    err_max = float((F - F_ref).norm(dim=0).max() / rms)
+    ap.close()
+    out = {"force_rel_l2": err, "force_max_over_rms": err_max, "x_rel_l2": x_err,
+           "samples": k, "ref_rel_residual": ref.stats["rel_residual_true"],
+           "ref_seconds": time.perf_counter() - t0}
+    log(f"check: {out}")
+    return out
+
+
+def cpu_baseline(system, budget_s, matvecs_per_step):
+    """The CPU oracle (FP64 all-pairs restatement) on a bounded sample of
+    target rows, extrapolated to one solve + force step of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    pos = system.pos.astype(np.float64)
+    n = system.n
+    rng = np.random.default_rng(1)
+    m = rng.normal(size=(n, 3)) * 0.5
+    rows = 64
+    t_field = t_force = 0.0
+    done_f = done_h = 0
+    t_start = time.perf_counter()
+    while time.perf_counter() - t_start < budget_s * 0.5:
+        sel = rng.choice(n, size=rows, replace=False)
+        t0 = time.perf_counter()
+        oracle.field_allpairs(pos[sel], pos, m, system.h, nthreads=threads)
+        t_field += time.perf_counter() - t0
+        done_f += rows
+    while time.perf_counter() - t_start < budget_s:
+        sel = rng.choice(n, size=rows // 4, replace=False)
+        t0 = time.perf_counter()
+        oracle.hessian_allpairs(pos[sel], pos, m, system.h, nthreads=threads)
+        t_force += time.perf_counter() - t0
+        done_h += rows // 4
+    per_field = t_field / done_f * n          # s per full field product
+    per_force = t_force / max(done_h, 1) * n  # s per full force product
+    step_ms = (matvecs_per_step * per_field + per_force) * 1e3
+    return {"value": step_ms, "unit": "ms/step", "cores": threads, "kind": "port",
+            "sample": (f"FP64 all-pairs oracle rows: {done_f} field and {done_h} force-tensor "
+                       f"target rows x {n} sources ({t_field + t_force:.1f} s), extrapolated to "
+                       f"{matvecs_per_step} field products + 1 force product")}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
+              f"design is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import magforce
+    from magforce import configs
+
+    log(f"bench: workload={args.workload} steps={args.steps} warmup={args.warmup}")
+    ctx = magforce.Context(0)
+    system = configs.WORKLOADS[args.workload]()
+    mode = {"fmm": magforce.MODE_FMM, "allpairs": magforce.MODE_ALLPAIRS}.get(args.mode)
+    if mode is None:
+        try:
+            p = magforce.Plan(ctx, to_dev(system.pos[:64]), system.h, mode=magforce.MODE_FMM)
+            p.close()
+            mode = magforce.MODE_FMM
+        except magforce.MagforceError:
+            mode = magforce.MODE_ALLPAIRS
+    mode_name = "fmm" if mode == magforce.MODE_FMM else "allpairs"
+    st = Stepper(ctx, system, mode, args.rtol)
+    for w in range(args.warmup):
+        st.step()
+        torch.cuda.synchronize()
+        log(f"warmup {w} done: {st.stats}")
+    dom = "p2p_field" if mode == magforce.MODE_FMM else "field"
+    ctx.reset_timing()
+    ctx.enable_timing(True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for k in range(args.steps):
+        st.step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = e0.elapsed_time(e1) / args.steps
+    log(f"timed: {args.steps} steps, {ms_step:.2f} ms/step (wall {wall / args.steps * 1e3:.2f})")
+    kms, kcnt = ctx.kernel_time(dom)
+    fms, fcnt = ctx.kernel_time("force")
+    ctx.enable_timing(False)
+    matvecs = st.stats["matvecs"]
+    if args.profile_only:
+        print(json.dumps({"ms_per_step": ms_step, "matvecs": matvecs}))
+        return
+
+    n = system.n
+    avg_ms = kms / max(kcnt, 1)
+    if mode == magforce.MODE_ALLPAIRS:
+        flops = float(n) * n * FIELD_FLOPS_PER_PAIR
+    else:
+        flops = float(st.plan_p2p_pairs()) * FIELD_FLOPS_PER_PAIR  # set once FMM exists
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    roofline = {"kernel": dom, "bound": "fp32-valu", "achieved": achieved,
+                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                "traffic": None, "avg_launch_ms": avg_ms, "launches": kcnt,
+                "flops_per_launch": flops}
+
+    allpairs = None if args.no_allpairs else measure_allpairs(ctx)
+    check = None if args.no_check else check_error(ctx, st, args.check_samples)
+    cpu = None if args.no_cpu else cpu_baseline(system, args.cpu_seconds, matvecs)
+
+    line = {
+        "metric": "ms/step (M-solve+force) at N=1M, err<=1e-5 vs FP64; Gpairs/s all-pairs",
+        "value": ms_step, "unit": "ms/step", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{system.name} (n={n}, h={system.h}, chi={system.chi}, "
+                               f"H=(0,0,1), CG rtol {args.rtol}, {mode_name})",
+                   "mode": mode_name, "n": n, "matvecs_per_step": matvecs,
+                   "solver": st.stats},
+        "error": check,
+        "allpairs": allpairs,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,224 @@
+/*
+ * mf_capi.h -- C ABI of the MI355X (gfx950) magnetic hot path.
+ *
+ * The reference has no C ABI: its boundary is the C++ class in
+ * magnetic_solver/magnetic_force.cuh:36-174.  This ABI is what that class
+ * (re-declared in include/magnetic_force.cuh) calls, and what any FFI
+ * (ctypes, cgo, JNI) binds.  Each entry point names the reference function
+ * it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status (mf_status); 0 = MF_OK.
+ *    mf_ctx_last_error()/mf_last_error() return the message of the last
+ *    failure (per context / per host thread).
+ *  - Particle arrays are device pointers in SoA form (x, y, z arrays of
+ *    float), the layout of the reference's cuda_abstract_vector
+ *    (cuda_abstract_vector.cuh:166-243: dev_x, dev_y, dev_z).
+ *  - Small parameter arrays (hconst[3], options, stats) are host memory.
+ *  - All work is enqueued on the context's stream (mf_ctx_set_stream);
+ *    functions that return host scalars synchronize that stream.
+ *  - The library never frees caller memory.  Handles are released by their
+ *    destroy function.  A context is not thread-safe: use one per thread.
+ *  - Units: x = B/mu0 per particle, m = kappa x, kappa = V chi/(1+chi)
+ *    (magnetic_force.cu:1016); fields in the B/mu0 form of the reference's
+ *    G (G m = H_ferro + M, magnetic_force.cuh:107); forces carry mu0 once.
+ */
+#ifndef MF_CAPI_H
+#define MF_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MF_ABI_VERSION 1
+
+typedef enum mf_status {
+  MF_OK = 0,
+  MF_ERR_INVALID_ARG = 1,
+  MF_ERR_HIP = 2,
+  MF_ERR_ALLOC = 3,
+  MF_ERR_NOT_CONVERGED = 4,
+  MF_ERR_UNSUPPORTED = 5,
+  MF_ERR_NO_DEVICE = 6
+} mf_status;
+
+typedef enum mf_precision {
+  MF_PREC_FP32 = 0,      /* FP32 pair math, FP32 tile partials, FP64 cross-tile sums */
+  MF_PREC_FP64 = 1       /* FP64 throughout (the GPU FP64 check path) */
+} mf_precision;
+
+typedef enum mf_mode {
+  MF_MODE_ALLPAIRS = 0,  /* reference "naive" mode, magnetic_force.cu:1200-1222 */
+  MF_MODE_FMM = 1        /* reference default FMM mode, magnetic_force.cu:1223-1270 */
+} mf_mode;
+
+typedef enum mf_method {
+  MF_SOLVE_CG = 0,       /* conjugate gradients on the SPD system (I - kappa G) x = b */
+  MF_SOLVE_GMRES = 1,    /* restarted GMRES(restart) */
+  MF_SOLVE_REF_NLCG = 2  /* the reference's nonlinear CG, magnetic_force.cu:1008-1095 */
+} mf_method;
+
+typedef struct mf_ctx mf_ctx;
+typedef struct mf_plan mf_plan;
+
+/* ---- context ------------------------------------------------------------- */
+int mf_abi_version(void);
+const char *mf_status_string(int status);
+/* Thread-local message of the last failure on this host thread. */
+const char *mf_last_error(void);
+
+/* Binds HIP device `device`, creates a default-priority stream. */
+int mf_ctx_create(int device, mf_ctx **out);
+int mf_ctx_destroy(mf_ctx *ctx);
+const char *mf_ctx_last_error(const mf_ctx *ctx);
+/* Process-wide context of the current HIP device, created on first use and
+ * kept until exit.  It runs on the HIP null stream, so its work is ordered
+ * with synchronous hipMemcpy calls, as the reference's kernels were with its
+ * cudaMemcpy calls (all on the legacy default stream).  Used by the C++
+ * drop-in classes in magnetic_force.cuh / cuda_abstract_vector.cuh. */
+int mf_ctx_get_default(mf_ctx **out);
+/* Use an existing hipStream_t (passed as void*); NULL restores the context's own. */
+int mf_ctx_set_stream(mf_ctx *ctx, void *hip_stream);
+void *mf_ctx_get_stream(mf_ctx *ctx);
+int mf_ctx_synchronize(mf_ctx *ctx);
+/* Per-kernel timing with HIP events on the context stream (off by default).
+ * mf_ctx_kernel_time returns the summed milliseconds and launch count of the
+ * named kernel family since the last reset ("field", "force", "hessian",
+ * "p2p_field", "p2p_force", "m2l", ...). */
+int mf_ctx_enable_timing(mf_ctx *ctx, int enable);
+int mf_ctx_kernel_time(mf_ctx *ctx, const char *name, double *ms_total, int64_t *launches);
+int mf_ctx_reset_timing(mf_ctx *ctx);
+
+/* ---- stateless all-pairs products (separate target and source sets) ------ */
+/* out_i = hconst + sum_j K(t_i - s_j) m_j.
+ * Replaces cuda_Apply_G (magnetic_force.cu:8-110, targets == sources) and
+ * cuda_ext_field_kernel (:120-237, external sources + constant field).
+ * K is DipoleBKernel (fmmtl_minimum/kernel/Dipole_B.kern:46-128).
+ * hconst (host, 3 floats) may be NULL.  precision: MF_PREC_FP32 only here. */
+int mf_field_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                      const float *tz, int64_t n_s, const float *sx, const float *sy,
+                      const float *sz, const float *mx, const float *my, const float *mz,
+                      float h, const float *hconst, float *ox, float *oy, float *oz);
+
+/* FP64 all-pairs field (device double SoA in and out): the GPU FP64 check
+ * path used to score FP32 results at sizes the CPU oracle cannot reach. */
+int mf_field_allpairs_f64(mf_ctx *ctx, int64_t n_t, const double *tx, const double *ty,
+                          const double *tz, int64_t n_s, const double *sx, const double *sy,
+                          const double *sz, const double *mx, const double *my,
+                          const double *mz, double h, const double *hconst, double *ox,
+                          double *oy, double *oz);
+
+/* Force-tensor field T_i = sum_j B(t_i - s_j)[m_j], 6 symmetric components
+ * written component-major to out6[6 * n_t] in the order xx, yy, zz, xy, xz, yz.
+ * Replaces cuda_ext_fitted_force_tensor_kernel (magnetic_force.cu:432-512);
+ * B is DipoleHessianKernel (fmmtl_minimum/kernel/Dipole_hessian.kern:19-278). */
+int mf_hessian_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                        const float *tz, int64_t n_s, const float *sx, const float *sy,
+                        const float *sz, const float *mx, const float *my, const float *mz,
+                        float h, float *out6);
+
+/* Force on one particle set: F_i = (sum_j B(x_i - x_j)[m_j] + Text_i) m_i.
+ * Replaces fitted_force_ofM_kernel (magnetic_force.cu:516-605).  text9 is a
+ * host array of 9 device pointers (the reference's dHextdx.{x,y,z},
+ * dHextdy.{x,y,z}, dHextdz.{x,y,z}) or NULL; Text carries mu0 once (the FMM
+ * path's convention, magnetic_force.cu:968-980). */
+int mf_force_allpairs(mf_ctx *ctx, int64_t n, const float *x, const float *y, const float *z,
+                      const float *mx, const float *my, const float *mz, float h,
+                      const float *const *text9, float *fx, float *fy, float *fz);
+
+/* FP64 force (device double SoA), the FP64 check path for the force. */
+int mf_force_allpairs_f64(mf_ctx *ctx, int64_t n, const double *x, const double *y,
+                          const double *z, const double *mx, const double *my,
+                          const double *mz, double h, double *fx, double *fy, double *fz);
+
+/* FP64 force at a subset of the particles: F_t for t = idx[k], k < n_t
+ * (idx: device int64 array), all n particles as sources.  Scores large runs
+ * on a seeded subsample. */
+int mf_force_subset_f64(mf_ctx *ctx, int64_t n, const double *x, const double *y,
+                        const double *z, const double *mx, const double *my, const double *mz,
+                        double h, int64_t n_t, const int64_t *idx, double *fx, double *fy,
+                        double *fz);
+
+/* ---- plans: one particle set, reused across products and solves ---------- */
+typedef struct mf_plan_opts {
+  int mode;          /* mf_mode */
+  int leaf_size;     /* FMM: target particles per leaf (0 = autotune) */
+  int order;         /* FMM: expansion order p, degrees 0..p-1 (0 = default) */
+  int m2l_precision; /* FMM: 0 = FP32, 1 = BF16 operands with FP32 accumulation */
+  int reserved[4];
+} mf_plan_opts;
+
+/* Replaces the construction of fmm_field_evaluator / fmm_hessian_evaluator
+ * (magnetic_force.cu:650-672, 838-860) and, for the all-pairs mode, the
+ * position upload of the constructor (:633).  Positions are copied. */
+int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const float *z,
+                   float h, const mf_plan_opts *opts, mf_plan **out);
+/* New positions (same or different n): setTargetpos, magnetic_force.cu:806-871. */
+int mf_plan_update_positions(mf_plan *plan, int64_t n, const float *x, const float *y,
+                             const float *z);
+int mf_plan_destroy(mf_plan *plan);
+int64_t mf_plan_size(const mf_plan *plan);
+
+/* out = G m (+ hconst): cuda_magnet_cgOptimizer::G, magnetic_force.cu:1196-1271. */
+int mf_plan_field(mf_plan *plan, const float *mx, const float *my, const float *mz,
+                  const float *hconst, float *ox, float *oy, float *oz);
+/* F = (B(m) + Text) m: evaluate_fitted_near_foece, magnetic_force.cu:890-997. */
+int mf_plan_force(mf_plan *plan, const float *mx, const float *my, const float *mz,
+                  const float *const *text9, float *fx, float *fy, float *fz);
+
+typedef struct mf_solve_opts {
+  int method;        /* mf_method */
+  int restart;       /* GMRES restart length (default 30) */
+  int maxit;         /* max matvecs (CG/GMRES) or max iterations (REF_NLCG) */
+  double rtol;       /* ||b - A x|| <= rtol ||b|| (CG/GMRES); RMS tol for REF_NLCG */
+  int refine_fp64;   /* mf_plan_solve_f64: max FP64 refinement sweeps (0 = 10) */
+  int true_residual; /* 1: recompute ||b - A x|| with a fresh product at exit */
+  int reserved[2];
+} mf_solve_opts;
+
+typedef struct mf_solve_stats {
+  int iterations;    /* inner iterations */
+  int matvecs;       /* G products */
+  int converged;
+  int refinements;
+  double rel_residual;      /* ||b - A x|| / ||b|| from the solver recurrences */
+  double rel_residual_true; /* recomputed with a fresh product at exit */
+} mf_solve_stats;
+
+/* Solve (I - kappa G) x = b in place (x holds the start).  Replaces
+ * cuda_magnet_cgOptimizer::solve / start / step_iteration
+ * (magnetic_force.cu:874-888, 1008-1095) with a device-resident solver. */
+int mf_plan_solve(mf_plan *plan, double kappa, const float *bx, const float *by,
+                  const float *bz, float *xx, float *xy, float *xz, const mf_solve_opts *opts,
+                  mf_solve_stats *stats);
+
+/* Mixed-precision iterative refinement: the residual r = b - A x is formed
+ * with the FP64 all-pairs product, corrections come from the FP32 plan
+ * solver, x is kept in FP64.  b and x are device double arrays laid out as
+ * one contiguous [3][n] block each (x component first).  Used to obtain the
+ * FP64 reference solution (rtol 1e-12) that FP32 and FMM results are scored
+ * against. */
+int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3n,
+                      const mf_solve_opts *opts, mf_solve_stats *stats);
+
+/* ---- fused SoA 3-vector utilities (cuda_abstract_vector.cuh:12-161, 502-563) */
+/* out = alpha a + beta b (any of a, b may alias out; b may be NULL => beta ignored) */
+int mf_vec_axpby(mf_ctx *ctx, int64_t n, float alpha, const float *ax, const float *ay,
+                 const float *az, float beta, const float *bx, const float *by,
+                 const float *bz, float *ox, float *oy, float *oz);
+/* out = a * b elementwise per component */
+int mf_vec_mul(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
+               const float *bx, const float *by, const float *bz, float *ox, float *oy,
+               float *oz);
+/* sum over i and components of a_i b_i, FP64 partials, deterministic order. */
+int mf_vec_dot(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
+               const float *bx, const float *by, const float *bz, double *result);
+/* x[i] = v for every i (one component array). */
+int mf_vec_fill(mf_ctx *ctx, int64_t n, float v, float *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

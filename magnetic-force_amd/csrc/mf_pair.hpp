@@ -1,0 +1,114 @@
+// mf_pair.hpp -- per-pair field and force-tensor coefficients for gfx950.
+//
+// Both pair kernels of the reference reduce to two radial scalars per pair:
+//
+// Field (fmmtl_minimum/kernel/Dipole_B.kern:46-104, == magnetic_force.cu:63-98)
+//   K(d) m = u (u.m)(w_avr - w_dens) - (w_avr/3 - w_dens) m,   u = d/|d|
+//          = c [ 3 alpha d (d.m) - beta m ],                    c = 1/(4 pi)
+// with (w~ = w / c)
+//   far  (q >= 2): alpha = |d|^-5,                 beta = |d|^-3
+//   q < 1       : alpha = (0.8 - 0.5 q)/h^5,       beta = (-8/3 + 4.8 q^2 - 2.5 q^3)/h^3
+//   1 <= q < 2  : alpha = (g(q)/r^3 - ((2-q)/h)^3)/(3 r^2),
+//                 beta  =  g(q)/(3 r^3) - ((2-q)/h)^3,
+//                 g(q) = -3 (q^6/6 - 6q^5/5 + 3q^4 - 8q^3/3 + 1/15)
+// The q < 1 forms are the reference's w_avr/w_dens polynomials with the
+// common factors cancelled exactly (w_avr - w_dens = sigma q^2 (0.6 - 0.375 q)),
+// which also removes the 0/0 at r = 0 (self term (2/3) sigma m).
+// g is evaluated re-centred at q = 1.5 (exact rational re-expansion), which
+// lowers its FP32 conditioning from 118 to 2.1.
+//
+// Force tensor (Dipole_hessian.kern:53-238, == magnetic_force.cu:239-427)
+//   B(d)[m]_ab = P (d_a m_b + m_a d_b + delta_ab d.m) + R d_a d_b (d.m)
+// scaled by s = 3 mu0/(4 pi):
+//   far  (q > 4) : P = |d|^-5,            R = -5 |d|^-7
+//   0 < q <= 4   : P = C1(q)/(s h^4 r),   R = (C2(q) - 3 C1(q))/(s h^4 r^3)
+//   q == 0       : 0
+// The rotation-frame product M U M^T of the reference equals this isotropic
+// form.  C1, C2 are the reference's fitted quartics, evaluated re-centred at
+// the middle of their interval (k + 0.5) so that FP32 Horner does not cancel
+// near q = 4 (absolute conditioning <= 2 on every interval).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mf {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kMu0 = 1.25663706e-6;  // the reference's truncated value
+
+// Field constants passed by value (kernel arguments live in SGPRs).
+struct FieldConsts {
+  float h;
+  float inv_h;
+  float near2;   // (2h)^2
+  float inv_h3;
+  float inv_h5;
+  float g[7];    // g(1.5 + t) = sum g[k] t^k
+};
+
+// Force-tensor constants; the curve table goes to LDS for per-lane lookup.
+struct HessConsts {
+  float h;
+  float inv_h;
+  float near2;       // (4h)^2
+  float curve_scale; // 1/(s h^4), s = 3 mu0/(4 pi)
+  // curve[k][0..4] = C1 re-centred at k+0.5 (ascending powers),
+  // curve[k][5..9] = C2 - 3 C1 re-centred likewise; [10..11] padding.
+  float curve[4][12];
+};
+
+FieldConsts make_field_consts(double h);
+HessConsts make_hess_consts(double h);
+
+// ---------------------------------------------------------------------------
+// Field pair coefficients (FP32).  Returns alpha, beta such that the pair adds
+// c (3 alpha d (d.m) - beta m).  `ir` = 1/|d| (may be inf at r2 == 0).
+__device__ __forceinline__ void field_near_coeffs(const FieldConsts &fc, float r2, float ir,
+                                                  float &alpha, float &beta) {
+  const float r = r2 * ir;                 // |d|; NaN when r2 == 0, handled below
+  const float q = (r2 > 0.f) ? r * fc.inv_h : 0.f;
+  // inner interval q < 1
+  const float a_in = fmaf(-0.5f, q, 0.8f) * fc.inv_h5;
+  const float q2 = q * q;
+  const float b_in = fmaf(q2, fmaf(-2.5f, q, 4.8f), -2.6666666666666667f) * fc.inv_h3;
+  // outer interval 1 <= q < 2
+  const float t = q - 1.5f;
+  float g = fc.g[6];
+  g = fmaf(g, t, fc.g[5]);
+  g = fmaf(g, t, fc.g[4]);
+  g = fmaf(g, t, fc.g[3]);
+  g = fmaf(g, t, fc.g[2]);
+  g = fmaf(g, t, fc.g[1]);
+  g = fmaf(g, t, fc.g[0]);
+  const float tmq = (2.f - q) * fc.inv_h;
+  const float wd = tmq * tmq * tmq;
+  const float ir3 = ir * ir * ir;
+  const float wa = g * ir3;
+  const float a_out = (wa - wd) * (ir * ir) * 0.33333333333333333f;
+  const float b_out = wa * 0.33333333333333333f - wd;
+  const bool inner = q < 1.f;
+  alpha = inner ? a_in : a_out;
+  beta = inner ? b_in : b_out;
+}
+
+// Force pair coefficients for the near field (q <= 4).  `tab` points at the
+// 4 x 12 curve table in LDS.  P, R as in the header comment (scaled by 1/s).
+__device__ __forceinline__ void hess_near_coeffs(const float *tab, float curve_scale,
+                                                 float inv_h, float r2, float ir, float &P,
+                                                 float &R) {
+  const float r = r2 * ir;
+  const float q = r * inv_h;
+  int k = static_cast<int>(ceilf(q)) - 1;
+  k = k < 0 ? 0 : (k > 3 ? 3 : k);
+  const float t = q - (static_cast<float>(k) + 0.5f);
+  const float4 *row = reinterpret_cast<const float4 *>(tab + 12 * k);
+  const float4 v0 = row[0], v1 = row[1], v2 = row[2];
+  // C1: v0.x v0.y v0.z v0.w v1.x ; D = C2 - 3 C1: v1.y v1.z v1.w v2.x v2.y
+  float c1 = fmaf(fmaf(fmaf(fmaf(v1.x, t, v0.w), t, v0.z), t, v0.y), t, v0.x);
+  float dd = fmaf(fmaf(fmaf(fmaf(v2.y, t, v2.x), t, v1.w), t, v1.z), t, v1.y);
+  const bool live = r2 > 0.f;
+  P = live ? c1 * curve_scale * ir : 0.f;
+  R = live ? dd * curve_scale * (ir * ir * ir) : 0.f;
+}
+
+}  // namespace mf

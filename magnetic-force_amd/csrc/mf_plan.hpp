@@ -1,0 +1,61 @@
+// mf_plan.hpp -- the particle-set plan shared by products and solves.
+#pragma once
+
+#include <memory>
+
+#include "mf_allpairs.hpp"
+#include "mf_internal.hpp"
+#include "mf_pair.hpp"
+
+namespace mf {
+struct FmmState;  // defined in mf_fmm.hip
+void destroy_fmm(FmmState *s);
+struct FmmDeleter {
+  void operator()(FmmState *s) const { destroy_fmm(s); }
+};
+}  // namespace mf
+
+struct mf_plan {
+  mf_ctx *ctx = nullptr;
+  mf_plan_opts opts{};
+  int64_t n = 0;
+  double h = 0.0;
+  mf::FieldConsts fc{};
+  mf::HessConsts hc{};
+  // All-pairs state: packed records (positions + current moments) and tile boxes.
+  mf::DeviceBuffer rec;
+  mf::DeviceBuffer boxes;
+  // Double copies of the positions for the FP64 check/refinement path (lazy).
+  mf::DeviceBuffer pos64;
+  bool pos64_valid = false;
+  std::unique_ptr<mf::FmmState, mf::FmmDeleter> fmm;
+};
+
+namespace mf {
+
+// out = G(scale * p) (+ hconst); p, out are SoA device arrays of plan->n.
+Status plan_apply_G(mf_plan *plan, float scale, const float *px, const float *py,
+                    const float *pz, const float *hconst, float *ox, float *oy, float *oz);
+// F = (B(m) + Text) m.
+Status plan_force(mf_plan *plan, const float *mx, const float *my, const float *mz,
+                  const float *const *text9, float *fx, float *fy, float *fz);
+// (Re)builds all position-dependent state.
+Status plan_set_positions(mf_plan *plan, int64_t n, const float *x, const float *y,
+                          const float *z);
+// Double positions [3][n] for the FP64 path.
+Status plan_pos64(mf_plan *plan, const double **px, const double **py, const double **pz);
+
+Status solve(mf_plan *plan, double kappa, const float *bx, const float *by, const float *bz,
+             float *xx, float *xy, float *xz, const mf_solve_opts &opts, mf_solve_stats *stats);
+Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by,
+                 const double *bz, double *xx, double *xy, double *xz,
+                 const mf_solve_opts &opts, mf_solve_stats *stats);
+
+// FMM (mf_fmm.hip).
+Status fmm_build(mf_plan *plan);
+Status fmm_apply_G(mf_plan *plan, float scale, const float *px, const float *py, const float *pz,
+                   const float *hconst, float *ox, float *oy, float *oz);
+Status fmm_force(mf_plan *plan, const float *mx, const float *my, const float *mz,
+                 const float *const *text9, float *fx, float *fy, float *fz);
+
+}  // namespace mf
