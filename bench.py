@@ -58,6 +58,12 @@ def parse():
     p.add_argument("--no-allpairs", action="store_true")
     p.add_argument("--profile-only", action="store_true",
                    help="only the timed steps (for rocprofv3 runs)")
+    p.add_argument("--copy-calib-mb", type=int, default=0,
+                   help="run one device-to-device copy of this many MiB first: a kernel of "
+                        "known traffic that scales the FETCH_SIZE/WRITE_SIZE counters")
+    p.add_argument("--traffic", default=None,
+                   help="JSON with measured HBM bytes per launch of the dominant kernel "
+                        "(from the rocprofv3 counter runs under profiles/)")
     return p.parse_args()
 
 
@@ -211,6 +217,14 @@ def main():
         except magforce.MagforceError:
             mode = magforce.MODE_ALLPAIRS
     mode_name = "fmm" if mode == magforce.MODE_FMM else "allpairs"
+    if args.copy_calib_mb > 0:
+        nbytes = args.copy_calib_mb << 20
+        a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nbytes} bytes read + {nbytes} bytes written")
+        del a, b
     st = Stepper(ctx, system, mode, args.rtol)
     for w in range(args.warmup):
         st.step()
@@ -245,9 +259,15 @@ def main():
     else:
         flops = float(st.plan_p2p_pairs()) * FIELD_FLOPS_PER_PAIR  # set once FMM exists
     achieved = flops / (avg_ms * 1e-3) / 1e12
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            t = json.load(f)
+        if t.get("workload") == system.name and t.get("kernel") == dom:
+            traffic = t["hbm_bytes_per_launch"]
     roofline = {"kernel": dom, "bound": "fp32-valu", "achieved": achieved,
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": None, "avg_launch_ms": avg_ms, "launches": kcnt,
+                "traffic": traffic, "avg_launch_ms": avg_ms, "launches": kcnt,
                 "flops_per_launch": flops}
 
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)

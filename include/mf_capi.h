@@ -79,7 +79,10 @@ const char *mf_ctx_last_error(const mf_ctx *ctx);
  * cudaMemcpy calls (all on the legacy default stream).  Used by the C++
  * drop-in classes in magnetic_force.cuh / cuda_abstract_vector.cuh. */
 int mf_ctx_get_default(mf_ctx **out);
-/* Use an existing hipStream_t (passed as void*); NULL restores the context's own. */
+/* Enqueue all further work on hip_stream (a hipStream_t passed as void*).
+ * NULL selects the HIP null stream.  A new context starts on a stream of its
+ * own (non-blocking, default priority); mf_ctx_get_stream returns the current
+ * one, so a caller can switch back later. */
 int mf_ctx_set_stream(mf_ctx *ctx, void *hip_stream);
 void *mf_ctx_get_stream(mf_ctx *ctx);
 int mf_ctx_synchronize(mf_ctx *ctx);

@@ -254,7 +254,7 @@ int mf_ctx_get_default(mf_ctx **out) {
 int mf_ctx_set_stream(mf_ctx *ctx, void *hip_stream) {
   if (!ctx) return MF_ERR_INVALID_ARG;
   (void)hipStreamSynchronize(ctx->stream);  // workspaces may be in use on the old stream
-  ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own_stream;
+  ctx->stream = static_cast<hipStream_t>(hip_stream);
   return MF_OK;
 }
 

@@ -126,7 +126,9 @@ def test_fp64_path_matches_oracle(ctx, oracle):
                                       dev(m, torch.float64), 2.0, hconst=[0, 0, 1])
     assert rel_err(host(got), want) < 1e-13
     F = magforce.force_allpairs_f64(ctx, dev(pos, torch.float64), dev(m, torch.float64), 2.0)
-    assert rel_err(host(F), oracle.force_allpairs(pos, m, 2.0)) < 1e-12
+    # the oracle keeps the reference's rotation frame with its 1e-10 pole
+    # patch (Dipole_hessian.kern:116); the GPU uses the exact isotropic form
+    assert rel_err(host(F), oracle.force_allpairs(pos, m, 2.0)) < 1e-9
 
 
 def test_golden_products(ctx):
@@ -176,3 +178,19 @@ def test_invalid_arguments_raise(ctx):
         magforce.Plan(ctx, dev(pos), -1.0)
     with pytest.raises(magforce.MagforceError):
         magforce.field_allpairs(ctx, dev(pos), dev(pos), dev(pos), 0.0)
+
+
+def test_results_ordered_with_torch_stream(ctx):
+    """Library kernels run on torch's current stream: a read-back issued
+    right after the call (no explicit synchronisation) sees the results."""
+    import magforce
+    import torch
+    pos = f32(jittered_block(40, 40, 20, seed=81))
+    m = f32(random_moments(pos.shape[0], seed=82))
+    plan = magforce.Plan(ctx, dev(pos), 2.0)
+    out = torch.full((3, pos.shape[0]), float("nan"), dtype=torch.float32, device="cuda")
+    plan.field(dev(m), out=out)
+    first = out.cpu()
+    assert not torch.isnan(first).any()
+    torch.cuda.synchronize()
+    assert torch.equal(first, out.cpu())
