@@ -254,10 +254,9 @@ def main():
 
     n = system.n
     avg_ms = kms / max(kcnt, 1)
-    if mode == magforce.MODE_ALLPAIRS:
-        flops = float(n) * n * FIELD_FLOPS_PER_PAIR
-    else:
-        flops = float(st.plan_p2p_pairs()) * FIELD_FLOPS_PER_PAIR  # set once FMM exists
+    info = st.plan.info()
+    # algorithmic FLOPs of one launch of the dominant kernel: direct pairs x 32
+    flops = float(info["p2p_field_pairs"]) * FIELD_FLOPS_PER_PAIR
     achieved = flops / (avg_ms * 1e-3) / 1e12
     traffic = None
     if args.traffic and os.path.exists(args.traffic):
@@ -269,6 +268,13 @@ def main():
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
                 "traffic": traffic, "avg_launch_ms": avg_ms, "launches": kcnt,
                 "flops_per_launch": flops}
+    if mode == magforce.MODE_FMM:
+        breakdown = {}
+        for k in ("p2m", "m2m", "m2l", "l2l", "p2p_field", "p2p_force"):
+            ms, cnt = ctx.kernel_time(k)
+            breakdown[k] = {"ms_total": ms / args.steps, "launches": cnt // max(args.steps, 1)}
+        roofline["per_step_breakdown"] = breakdown
+        roofline["plan"] = info
 
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)
     check = None if args.no_check else check_error(ctx, st, args.check_samples)

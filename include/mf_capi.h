@@ -164,6 +164,23 @@ int mf_plan_update_positions(mf_plan *plan, int64_t n, const float *x, const flo
 int mf_plan_destroy(mf_plan *plan);
 int64_t mf_plan_size(const mf_plan *plan);
 
+/* Work counts of a plan, for throughput and roofline accounting.  All-pairs
+ * plans report n^2 near pairs and no boxes. */
+typedef struct mf_plan_info {
+  int64_t n;
+  int mode;
+  int order;                /* FMM expansion order (0 for all-pairs) */
+  int depth;                /* octree leaf level */
+  int reserved0;
+  int64_t leaves;
+  int64_t boxes;
+  int64_t p2p_field_pairs;  /* particle pairs evaluated directly per field product */
+  int64_t p2p_force_pairs;  /* ... per force product */
+  int64_t m2l_field_pairs;  /* box pairs translated per field product */
+  int64_t m2l_force_pairs;
+} mf_plan_info;
+int mf_plan_get_info(const mf_plan *plan, mf_plan_info *info);
+
 /* out = G m (+ hconst): cuda_magnet_cgOptimizer::G, magnetic_force.cu:1196-1271. */
 int mf_plan_field(mf_plan *plan, const float *mx, const float *my, const float *mz,
                   const float *hconst, float *ox, float *oy, float *oz);

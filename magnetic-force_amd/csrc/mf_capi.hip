@@ -476,6 +476,20 @@ int mf_plan_destroy(mf_plan *plan) {
 
 int64_t mf_plan_size(const mf_plan *plan) { return plan ? plan->n : -1; }
 
+int mf_plan_get_info(const mf_plan *plan, mf_plan_info *info) {
+  if (!plan || !info) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan/info is null"));
+  *info = mf_plan_info{};
+  info->n = plan->n;
+  info->mode = plan->opts.mode;
+  if (plan->opts.mode == MF_MODE_FMM && plan->fmm) {
+    mf::fmm_info(plan, info);
+  } else {
+    info->p2p_field_pairs = plan->n * plan->n;
+    info->p2p_force_pairs = plan->n * plan->n;
+  }
+  return MF_OK;
+}
+
 int mf_plan_field(mf_plan *plan, const float *mx, const float *my, const float *mz,
                   const float *hconst, float *ox, float *oy, float *oz) {
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));

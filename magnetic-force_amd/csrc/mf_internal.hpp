@@ -121,6 +121,8 @@ struct mf_ctx {
   std::vector<hipEvent_t> event_pool;
   // Named workspaces, grown on demand and kept for the context's lifetime.
   std::map<std::string, mf::DeviceBuffer> ws;
+  // Long-lived per-context objects (operator tables), keyed by name.
+  std::map<std::string, std::shared_ptr<void>> cache;
   // Pinned host staging for scalar read-backs.
   double *host_scalars = nullptr;
   int num_cu = 256;

@@ -57,5 +57,6 @@ Status fmm_apply_G(mf_plan *plan, float scale, const float *px, const float *py,
                    const float *hconst, float *ox, float *oy, float *oz);
 Status fmm_force(mf_plan *plan, const float *mx, const float *my, const float *mz,
                  const float *const *text9, float *fx, float *fy, float *fz);
+void fmm_info(const mf_plan *plan, mf_plan_info *info);
 
 }  // namespace mf

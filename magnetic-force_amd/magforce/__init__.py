@@ -54,6 +54,17 @@ class PlanOpts(ctypes.Structure):
                 ("m2l_precision", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("mode", ctypes.c_int), ("order", ctypes.c_int),
+                ("depth", ctypes.c_int), ("reserved0", ctypes.c_int), ("leaves", ctypes.c_int64),
+                ("boxes", ctypes.c_int64), ("p2p_field_pairs", ctypes.c_int64),
+                ("p2p_force_pairs", ctypes.c_int64), ("m2l_field_pairs", ctypes.c_int64),
+                ("m2l_force_pairs", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved0"}
+
+
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("maxit", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("refine_fp64", ctypes.c_int),
@@ -106,6 +117,7 @@ _SIGS = {
     "mf_plan_update_positions": ([_vp, _i64, _vp, _vp, _vp], _i),
     "mf_plan_destroy": ([_vp], _i),
     "mf_plan_size": ([_vp], _i64),
+    "mf_plan_get_info": ([_vp, ctypes.POINTER(PlanInfo)], _i),
     "mf_plan_field": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mf_plan_force": ([_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mf_plan_solve": ([_vp, _d, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(SolveOpts),
@@ -353,6 +365,11 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+    def info(self) -> dict:
+        pi = PlanInfo()
+        _check(lib().mf_plan_get_info(self.h_plan, ctypes.byref(pi)), self.ctx.h)
+        return pi.as_dict()
 
     def update_positions(self, pos):
         p = _soa(pos)

@@ -1,0 +1,134 @@
+"""GPU FMM (MODE_FMM plans) against the FP64 oracle: field, force, solve,
+degenerate trees.  Tolerances per expansion order p (near-set |o|^2 <= 6,
+leaves of side 2h): field/force relative L2 error <= 2e-6 at p = 10,
+3e-5 at p = 8, 5e-4 at p = 6."""
+import numpy as np
+import pytest
+
+from conftest import jittered_block, random_moments, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {6: 5e-4, 8: 3e-5, 10: 2e-6}
+
+
+def dev(a, dtype=None):
+    import torch
+    return torch.tensor(np.ascontiguousarray(np.asarray(a).T),
+                        dtype=dtype or torch.float32, device="cuda").contiguous()
+
+
+def host(t):
+    return t.double().cpu().numpy().T
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def slab(nx, ny, nz, seed=0):
+    return f32(jittered_block(nx, ny, nz, seed=seed))
+
+
+@pytest.fixture(scope="module")
+def slab_case(oracle):
+    pos = slab(40, 36, 10, seed=5)
+    m = f32(random_moments(pos.shape[0], seed=6, scale=0.4))
+    field = oracle.field_allpairs(pos, pos, m, 2.0)
+    force = oracle.force_allpairs(pos, m, 2.0)
+    return pos, m, field, force
+
+
+@pytest.mark.parametrize("order", [6, 8, 10])
+def test_fmm_field_matches_oracle(ctx, slab_case, order):
+    import magforce
+    pos, m, field, _ = slab_case
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order)
+    got = host(plan.field(dev(m)))
+    assert rel_err(got, field) < TOL[order]
+
+
+@pytest.mark.parametrize("order", [6, 8, 10])
+def test_fmm_force_matches_oracle(ctx, slab_case, order):
+    import magforce
+    pos, m, _, force = slab_case
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order)
+    got = host(plan.force(dev(m)))
+    assert rel_err(got, force) < TOL[order]
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (2, 2, 2), (3, 7, 2), (9, 9, 9)])
+def test_fmm_small_and_shallow_trees(ctx, oracle, shape):
+    """Trees of one or two levels have no far field (exact near field only);
+    the 9^3 cube already has a level-2 interaction list."""
+    import magforce
+    pos = slab(*shape, seed=11)
+    m = f32(random_moments(pos.shape[0], seed=12))
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < 5e-6
+    F = host(plan.force(dev(m)))
+    if pos.shape[0] > 1:
+        assert rel_err(F, oracle.force_allpairs(pos, m, 2.0)) < 5e-6
+
+
+def test_fmm_sparse_deep_tree(ctx, oracle):
+    """Widely spread clusters: many levels with few boxes each."""
+    import magforce
+    rng = np.random.default_rng(3)
+    centres = rng.uniform(0, 600, size=(12, 3))
+    pos = f32(np.concatenate([c + rng.normal(size=(150, 3)) * 2.0 for c in centres]))
+    m = f32(random_moments(pos.shape[0], seed=4))
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < 2e-6
+    assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < 2e-6
+
+
+def test_fmm_hconst_and_update_positions(ctx, oracle):
+    import magforce
+    pos = slab(20, 20, 6, seed=21)
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    pos2 = slab(17, 23, 8, seed=22) + 3.3
+    m2 = f32(random_moments(pos2.shape[0], seed=23))
+    plan.update_positions(dev(pos2))
+    got = host(plan.field(dev(m2), hconst=[0.0, 0.5, 1.0]))
+    want = oracle.field_allpairs(pos2, pos2, m2, 2.0, hconst=[0.0, 0.5, 1.0])
+    assert rel_err(got, want) < 2e-6
+
+
+def test_fmm_solve_and_force_vs_oracle(ctx, oracle):
+    """The 8192-particle lattice solved with FMM products (CG, 1e-6), force
+    by FMM, scored against the FP64 oracle solution (1e-12)."""
+    import magforce
+    from magforce import configs
+    s = configs.lattice8k()
+    pos = s.pos.astype(np.float64)
+    b = np.tile(s.hconst, (s.n, 1))
+    x_ref, _, _ = oracle.cg_solve(pos, s.h, s.kappa, b, rtol=1e-12)
+    F_ref = oracle.force_allpairs(pos, s.kappa * x_ref, s.h)
+    plan = magforce.Plan(ctx, dev(pos), s.h, mode=magforce.MODE_FMM)
+    res = plan.solve(s.kappa, dev(b), rtol=1e-6)
+    assert res.stats["converged"] == 1
+    x = host(res.x)
+    assert rel_err(x, x_ref) < 5e-6
+    F = host(plan.force(dev(s.kappa * x)))
+    assert rel_err(F, F_ref) < 1e-5
+
+
+def test_fmm_matches_fp64_gpu_at_100k(ctx):
+    """A 100k-particle slab with a random layer above it, against the GPU FP64
+    all-pairs field (itself pinned to the CPU oracle in test_gpu_allpairs)."""
+    import magforce
+    import torch
+    rng = np.random.default_rng(9)
+    lat = jittered_block(64, 64, 22, seed=9)
+    top = np.column_stack([rng.uniform(0, 64, 4000), rng.uniform(0, 64, 4000),
+                           rng.uniform(22, 40, 4000)])
+    pos = f32(np.concatenate([lat, top]))
+    m = f32(random_moments(pos.shape[0], seed=10, scale=0.5))
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    got = host(plan.field(dev(m)))
+    want = host(magforce.field_allpairs_f64(ctx, dev(pos, torch.float64), dev(pos, torch.float64),
+                                            dev(m, torch.float64), 2.0))
+    # random moments: the field is small where it cancels, so the relative
+    # error runs above that of a solved (smooth) magnetisation
+    assert rel_err(got, want) < 5e-6
