@@ -90,7 +90,6 @@ class Stepper:
             self.plan = magforce.Plan(self.ctx, self.pos, self.s.h, mode=self.mode)
         else:
             self.plan.update_positions(self.pos)
-        self.x.zero_()
         res = self.plan.solve(self.s.kappa, self.b, x0=None, rtol=self.rtol)
         self.x = res.x
         self.stats = res.stats
@@ -270,7 +269,7 @@ def main():
                 "flops_per_launch": flops}
     if mode == magforce.MODE_FMM:
         breakdown = {}
-        for k in ("p2m", "m2m", "m2l", "l2l", "p2p_field", "p2p_force"):
+        for k in ("p2m", "m2m", "m2l", "l2l", "l2p_field", "p2p_field", "l2p_force", "p2p_force"):
             ms, cnt = ctx.kernel_time(k)
             breakdown[k] = {"ms_total": ms / args.steps, "launches": cnt // max(args.steps, 1)}
         roofline["per_step_breakdown"] = breakdown

@@ -195,7 +195,9 @@ typedef struct mf_solve_opts {
   double rtol;       /* ||b - A x|| <= rtol ||b|| (CG/GMRES); RMS tol for REF_NLCG */
   int refine_fp64;   /* mf_plan_solve_f64: max FP64 refinement sweeps (0 = 10) */
   int true_residual; /* 1: recompute ||b - A x|| with a fresh product at exit */
-  int reserved[2];
+  int zero_start;    /* 1: start from x = 0 (x is overwritten); the first residual
+                        is b itself, which saves one product */
+  int reserved[1];
 } mf_solve_opts;
 
 typedef struct mf_solve_stats {

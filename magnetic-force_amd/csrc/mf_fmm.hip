@@ -25,10 +25,10 @@
 //    boxes of one (level, parity) class and walks that class's offset list;
 //    per offset it gathers the 64 source multipoles into LDS and multiplies by
 //    the offset's matrix, accumulating the 64 local expansions in registers.
-//  * L2P is fused into the near-field kernel: one wave per leaf evaluates the
-//    far field from the leaf's local expansion (scalar loads), then adds the
-//    near field over the leaf's near list with the all-pairs inner loops, and
-//    writes the result back in the caller's particle order.
+//  * Leaf pass: L2P (one wave per leaf, local expansion through scalar loads)
+//    writes H_const + far field in the caller's particle order; the near-field
+//    kernel then adds the sums over the leaf's near boxes, whose particle
+//    ranges are merged into contiguous runs at plan build.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -367,14 +367,23 @@ __global__ __launch_bounds__(64) void l2l_kernel(int nchild, int child_base,
 }
 
 // ---------------------------------------------------------------------------
-// M2L on MFMA.  Workgroup = KP/32 waves, 64 target boxes of one class.
-// Wave w owns output rows [32 w, 32 w + 32) for all 64 targets:
-// 2 row blocks x 4 column blocks of 16x16, K stepped by 4.
+// M2L on MFMA.  A workgroup owns a tile of up to kTileT = 128 target boxes of
+// one (level, parity) class and walks that class's offset list.  Per offset
+// it gathers the tile's source multipoles into LDS (skipping the offset when
+// no target of the tile has a source there) and multiplies them by the
+// offset's KP x KP matrix, accumulating the local expansions in registers.
+// KP/32 waves: wave w owns rows [32 w, 32 w + 32) of all 64 columns:
+// 2 x 4 blocks of 16x16, K stepped by 4.
 // MFMA 16x16x4 f32 operand layout (wave64): lane l supplies A[l % 16][l / 16]
 // and B[l / 16][l % 16]; D[4 (l / 16) + r][l % 16] = acc[r].
+// Workgroups are remapped so that each XCD (round-robin dispatch) walks a
+// contiguous range of tiles: neighbouring tiles share classes, offset
+// matrices and source multipoles, which then stay in that XCD's L2.
+constexpr int kTileT = 64;
+
 struct M2LTile {
   int first;   // position in the class-sorted box list
-  int count;   // <= 64
+  int count;   // <= kTileT
   int slots;   // row of the slot -> table map (kind * 8 + parity)
   int nslot;
 };
@@ -384,17 +393,23 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
                                                      float *__restrict__ Lt,
                                                      const float *__restrict__ T,
                                                      const M2LTile *__restrict__ tiles,
+                                                     int ntiles,
                                                      const int32_t *__restrict__ sorted_boxes,
                                                      const int32_t *__restrict__ src_slots,
                                                      int src_ld,
                                                      const int32_t *__restrict__ slot_table,
                                                      int slot_ld) {
-  __shared__ float sB[64][KP + 4];
-  __shared__ int s_tgt[64];
-  const M2LTile tile = tiles[blockIdx.x];
+  constexpr int NT = 2 * KP;  // threads: KP/32 waves
+  __shared__ float sB[kTileT][KP + 4];
+  __shared__ int s_tgt[kTileT];
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+  if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  const M2LTile tile = tiles[tile_id];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  if (tid < 64) s_tgt[tid] = tid < tile.count ? sorted_boxes[tile.first + tid] : -1;
+  const int wr = wave;
+  for (int j = tid; j < kTileT; j += NT) s_tgt[j] = j < tile.count ? sorted_boxes[tile.first + j] : -1;
   __syncthreads();
   typedef float f4 __attribute__((ext_vector_type(4)));
   f4 acc[2][4];
@@ -403,26 +418,31 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
   const int32_t *__restrict__ smap = slot_table + static_cast<int64_t>(tile.slots) * slot_ld;
-  const int row0 = 32 * wave + (lane & 15);
+  const int row0 = 32 * wr + (lane & 15);
+  const int col0 = lane & 15;
   const int kq = lane >> 4;
   for (int slot = 0; slot < tile.nslot; ++slot) {
     const int tix = smap[slot];
     if (tix < 0) continue;
     __syncthreads();  // previous slot's B fully consumed
-    for (int e = tid; e < 64 * (KP / 4); e += 2 * KP) {
+    int any = 0;
+    for (int e = tid; e < kTileT * (KP / 4); e += NT) {
       const int j = e / (KP / 4), q = e % (KP / 4);
       const int tgt = s_tgt[j];
       float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
       if (tgt >= 0) {
         const int src = src_slots[static_cast<int64_t>(tgt) * src_ld + slot];
-        if (src >= 0) v = reinterpret_cast<const float4 *>(Mt + static_cast<int64_t>(src) * KP)[q];
+        if (src >= 0) {
+          v = reinterpret_cast<const float4 *>(Mt + static_cast<int64_t>(src) * KP)[q];
+          any = 1;
+        }
       }
       sB[j][4 * q] = v.x;
       sB[j][4 * q + 1] = v.y;
       sB[j][4 * q + 2] = v.z;
       sB[j][4 * q + 3] = v.w;
     }
-    __syncthreads();
+    if (!__syncthreads_or(any)) continue;  // no source at this offset for the whole tile
     const float *__restrict__ A = T + static_cast<int64_t>(tix) * KP * KP;
 #pragma unroll 4
     for (int k0 = 0; k0 < KP; k0 += 4) {
@@ -431,7 +451,7 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
       const float a1 = A[k * KP + row0 + 16];
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
-        const float bv = sB[16 * cb + (lane & 15)][k];
+        const float bv = sB[col0 + 16 * cb][k];
         acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][cb], 0, 0, 0);
         acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][cb], 0, 0, 0);
       }
@@ -440,13 +460,13 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
   // write the local expansions (every target of the tile is written once)
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
-    const int tgt = s_tgt[16 * cb + (lane & 15)];
+    const int tgt = s_tgt[col0 + 16 * cb];
     if (tgt < 0) continue;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 32 * wave + 16 * rb + 4 * kq + r;
+        const int row = 32 * wr + 16 * rb + 4 * kq + r;
         Lt[static_cast<int64_t>(tgt) * KP + row] = acc[rb][cb][r];
       }
   }
@@ -545,113 +565,252 @@ __device__ __forceinline__ void l2p_hess(const float *__restrict__ Lt, float x, 
   H[5] = Bi;
 }
 
-__device__ __forceinline__ float uniform_fl(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+// ---------------------------------------------------------------------------
+// Leaf kernels.  The far field (L2P) and the near field (P2P) are separate
+// launches: L2P writes H_const + far field (or the far force + external
+// tensor term) in the caller's particle order, then the near-field kernel adds
+// its sums.  Apart, the near-field kernel needs a third of the registers the
+// solid harmonics of L2P take, and runs at twice the occupancy.
+
+// Near ranges of a leaf: the particle ranges of its near boxes, with boxes of
+// consecutive index merged (leaf b + 1 holds the particles right after leaf
+// b's, so a run of consecutive leaf indices is one contiguous range).  One
+// wave per leaf.  Slots [0, nmix) form class 0 and [nmix, nslot) class 1;
+// ranges never cross classes and keep slot order, so the summation order is
+// fixed.  rng[leaf * nslot + r] = {first, end}; nr[leaf] = {class-0 ranges,
+// class-1 ranges} (class 1 follows class 0 in rng).
+__device__ __forceinline__ bool slot_has(const int *__restrict__ s_box, int lo, int hi, int box) {
+  if (box < 0) return false;
+  for (int k = lo; k < hi; ++k)
+    if (s_box[k] == box) return true;
+  return false;
 }
 
-// ---------------------------------------------------------------------------
-// Fused L2P + near field, field version.  One wave per leaf (blockDim = 64).
+__global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_t *__restrict__ list,
+                                                         int nslot, int nmix,
+                                                         const int32_t *__restrict__ first,
+                                                         const int32_t *__restrict__ count,
+                                                         int2 *__restrict__ rng,
+                                                         int2 *__restrict__ nr) {
+  __shared__ int s_box[kNearH];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t *__restrict__ l = list + static_cast<int64_t>(b) * nslot;
+  for (int k = lane; k < nslot; k += 64) s_box[k] = l[k];
+  __syncthreads();
+  int2 *__restrict__ out = rng + static_cast<int64_t>(b) * nslot;
+  int pos = 0, n0 = 0;
+  for (int cls = 0; cls < 2; ++cls) {
+    const int lo = cls ? nmix : 0, hi = cls ? nslot : nmix;
+    for (int k0 = lo; k0 < hi; k0 += 64) {
+      const int k = k0 + lane;
+      bool is_start = false;
+      int j0 = 0, j1 = 0;
+      if (k < hi) {
+        const int sb = s_box[k];
+        if (sb >= 0 && !slot_has(s_box, lo, hi, sb - 1)) {
+          int e = sb;
+          while (slot_has(s_box, lo, hi, e + 1)) ++e;
+          is_start = true;
+          j0 = first[sb];
+          j1 = first[e] + count[e];
+        }
+      }
+      const uint64_t mask = __ballot(is_start);
+      const int before = __popcll(mask & ((1ULL << lane) - 1ULL));
+      if (is_start) out[pos + before] = make_int2(j0, j1);
+      pos += __popcll(mask);
+    }
+    if (cls == 0) n0 = pos;
+  }
+  if (lane == 0) nr[b] = make_int2(n0, pos - n0);
+}
+
+// L2P, field: out[perm] = H_const + far field.  One wave per leaf.
 template <int P, int KP>
-__global__ __launch_bounds__(64) void leaf_field_kernel(
+__global__ __launch_bounds__(64) void l2p_field_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const int32_t *__restrict__ near, const float *__restrict__ Lt,
-    int leaf_base, int use_far, double lox, double loy, double loz, double s, FieldConsts fc,
-    float hc0, float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy,
-    float *__restrict__ oz) {
+    const int32_t *__restrict__ perm, const float *__restrict__ Lt, double lox, double loy,
+    double loz, double s, float hc0, float hc1, float hc2, float *__restrict__ ox,
+    float *__restrict__ oy, float *__restrict__ oz) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const I3 c = demorton(keys[b]);
   const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
   const float inv_s = static_cast<float>(1.0 / s);
-  const float far_scale = static_cast<float>(-1.0 / (4.0 * kPi) / (s * s));
+  const double far_scale = -1.0 / (4.0 * kPi) / (s * s);
   const int s0 = start[b], cnt = count[b];
-  const float *__restrict__ L = Lt + static_cast<int64_t>(leaf_base + b) * KP;
-  const int32_t *__restrict__ nl = near + static_cast<int64_t>(b) * kNearF;
+  const float *__restrict__ L = Lt + static_cast<int64_t>(b) * KP;
   for (int t0 = 0; t0 < cnt; t0 += 64) {
     const int t = t0 + lane;
-    const bool valid = t < cnt;
-    const SrcRec me = rec[s0 + (valid ? t : cnt - 1)];
-    const float tx = me.x, ty = me.y, tz = me.z;
-    double acc0 = hc0, acc1 = hc1, acc2 = hc2;
-    if (use_far) {
-      float gx, gy, gz;
-      l2p_grad<P>(L, static_cast<float>(tx - cx) * inv_s, static_cast<float>(ty - cy) * inv_s,
-                  static_cast<float>(tz - cz) * inv_s, gx, gy, gz);
-      acc0 += far_scale * gx;
-      acc1 += far_scale * gy;
-      acc2 += far_scale * gz;
-    }
-    for (int k = 0; k < kNearF; ++k) {
-      const int sb = nl[k];
-      if (sb < 0) continue;
-      const int j0 = start[sb], j1 = j0 + count[sb];
-      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
-      if (k >= kNearFMixed) {
-        for (int j = j0; j < j1; ++j) {
-          const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
-          const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
-          const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const float ir = __builtin_amdgcn_rsqf(r2);
-          const float ir2 = ir * ir;
-          const float ir3 = ir2 * ir;
-          const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-          const float s1 = dm * (ir3 * ir2);
-          a1x = fmaf(dx, s1, a1x);
-          a1y = fmaf(dy, s1, a1y);
-          a1z = fmaf(dz, s1, a1z);
-          a2x = fmaf(mx, ir3, a2x);
-          a2y = fmaf(my, ir3, a2y);
-          a2z = fmaf(mz, ir3, a2z);
-        }
-      } else {
-        for (int j = j0; j < j1; ++j) {
-          const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
-          const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
-          const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
-          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const float ir = __builtin_amdgcn_rsqf(r2);
-          const float ir2 = ir * ir;
-          float beta = ir2 * ir;
-          float alpha = beta * ir2;
-          if (r2 < fc.near2) field_near_coeffs(fc, r2, ir, alpha, beta);
-          const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-          const float s1 = dm * alpha;
-          a1x = fmaf(dx, s1, a1x);
-          a1y = fmaf(dy, s1, a1y);
-          a1z = fmaf(dz, s1, a1z);
-          a2x = fmaf(mx, beta, a2x);
-          a2y = fmaf(my, beta, a2y);
-          a2z = fmaf(mz, beta, a2z);
-        }
-      }
-      const double cf = 1.0 / (4.0 * kPi);
-      acc0 += cf * (3.0 * static_cast<double>(a1x) - static_cast<double>(a2x));
-      acc1 += cf * (3.0 * static_cast<double>(a1y) - static_cast<double>(a2y));
-      acc2 += cf * (3.0 * static_cast<double>(a1z) - static_cast<double>(a2z));
-    }
-    if (valid) {
+    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+    float gx, gy, gz;
+    l2p_grad<P>(L, static_cast<float>(me.x - cx) * inv_s, static_cast<float>(me.y - cy) * inv_s,
+                static_cast<float>(me.z - cz) * inv_s, gx, gy, gz);
+    if (t < cnt) {
       const int o = perm[s0 + t];
-      ox[o] = static_cast<float>(acc0);
-      oy[o] = static_cast<float>(acc1);
-      oz[o] = static_cast<float>(acc2);
+      ox[o] = static_cast<float>(hc0 + far_scale * gx);
+      oy[o] = static_cast<float>(hc1 + far_scale * gy);
+      oz[o] = static_cast<float>(hc2 + far_scale * gz);
     }
   }
 }
 
-// Fused L2P (force tensor) + near-field force.  One wave per leaf.
+// Near-field source loops over one contiguous source range.  Sources are
+// read with uniform (scalar) loads; each lane owns one target.
+__device__ __forceinline__ void field_range_far(const SrcRec *__restrict__ rec, int j0, int j1,
+                                                float tx, float ty, float tz, float &a1x,
+                                                float &a1y, float &a1z, float &a2x, float &a2y,
+                                                float &a2z) {
+#pragma unroll 2
+  for (int j = j0; j < j1; ++j) {
+    const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
+    const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
+    const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    const float ir3 = ir2 * ir;
+    const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+    const float s1 = dm * (ir3 * ir2);
+    a1x = fmaf(dx, s1, a1x);
+    a1y = fmaf(dy, s1, a1y);
+    a1z = fmaf(dz, s1, a1z);
+    a2x = fmaf(mx, ir3, a2x);
+    a2y = fmaf(my, ir3, a2y);
+    a2z = fmaf(mz, ir3, a2z);
+  }
+}
+
+__device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec, int j0, int j1,
+                                                  float tx, float ty, float tz,
+                                                  const FieldConsts &fc, float &a1x, float &a1y,
+                                                  float &a1z, float &a2x, float &a2y,
+                                                  float &a2z) {
+#pragma unroll 2
+  for (int j = j0; j < j1; ++j) {
+    const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
+    const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
+    const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    float beta = ir2 * ir;
+    float alpha = beta * ir2;
+    if (r2 < fc.near2) field_near_coeffs(fc, r2, ir, alpha, beta);
+    const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+    const float s1 = dm * alpha;
+    a1x = fmaf(dx, s1, a1x);
+    a1y = fmaf(dy, s1, a1y);
+    a1z = fmaf(dz, s1, a1z);
+    a2x = fmaf(mx, beta, a2x);
+    a2y = fmaf(my, beta, a2y);
+    a2z = fmaf(mz, beta, a2z);
+  }
+}
+
+// P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
+// writes H_const + near (trees without a far field).  One wave per leaf.
+__global__ __launch_bounds__(64) void p2p_field_kernel(
+    int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm, const int2 *__restrict__ rng,
+    const int2 *__restrict__ nr, int nslot, int accumulate, FieldConsts fc, float hc0, float hc1,
+    float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int s0 = start[b], cnt = count[b];
+  const int2 nrb = nr[b];
+  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
+  const double cf = 1.0 / (4.0 * kPi);
+  for (int t0 = 0; t0 < cnt; t0 += 64) {
+    const int t = t0 + lane;
+    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+    const float tx = me.x, ty = me.y, tz = me.z;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (int r = 0; r < nrb.x + nrb.y; ++r) {
+      const int2 q = rg[r];
+      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
+      if (r < nrb.x)
+        field_range_mixed(rec, q.x, q.y, tx, ty, tz, fc, a1x, a1y, a1z, a2x, a2y, a2z);
+      else
+        field_range_far(rec, q.x, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
+      acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
+      acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
+      acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
+    }
+    if (t < cnt) {
+      const int o = perm[s0 + t];
+      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
+      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
+      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
+      ox[o] = static_cast<float>(b0 + cf * acc0);
+      oy[o] = static_cast<float>(b1 + cf * acc1);
+      oz[o] = static_cast<float>(b2 + cf * acc2);
+    }
+  }
+}
+
 struct Text9 {
   const float *p[9];
 };
+
+// L2P, force: F[perm] = (far tensor + T_ext) . m.  One wave per leaf.
 template <int P, int KP>
-__global__ __launch_bounds__(64) void leaf_force_kernel(
+__global__ __launch_bounds__(64) void l2p_force_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const int32_t *__restrict__ near, const float *__restrict__ Lt,
-    int leaf_base, int use_far, double lox, double loy, double loz, double s, HessConsts hc,
-    Text9 text, int has_text, float *__restrict__ fx, float *__restrict__ fy,
-    float *__restrict__ fz) {
+    const int32_t *__restrict__ perm, const float *__restrict__ Lt, int use_far, double lox,
+    double loy, double loz, double s, Text9 text, int has_text, float *__restrict__ fx,
+    float *__restrict__ fy, float *__restrict__ fz) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  const I3 c = demorton(keys[b]);
+  const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
+  const float inv_s = static_cast<float>(1.0 / s);
+  const double far_scale = -kMu0 / (4.0 * kPi) / (s * s * s);
+  const int s0 = start[b], cnt = count[b];
+  const float *__restrict__ L = Lt + static_cast<int64_t>(b) * KP;
+  for (int t0 = 0; t0 < cnt; t0 += 64) {
+    const int t = t0 + lane;
+    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+    const float nx = me.mx, ny = me.my, nz = me.mz;
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    if (use_far) {
+      float Hs[6];
+      l2p_hess<P>(L, static_cast<float>(me.x - cx) * inv_s, static_cast<float>(me.y - cy) * inv_s,
+                  static_cast<float>(me.z - cz) * inv_s, Hs);
+      F0 = far_scale * (static_cast<double>(Hs[0]) * nx + static_cast<double>(Hs[3]) * ny +
+                        static_cast<double>(Hs[4]) * nz);
+      F1 = far_scale * (static_cast<double>(Hs[3]) * nx + static_cast<double>(Hs[1]) * ny +
+                        static_cast<double>(Hs[5]) * nz);
+      F2 = far_scale * (static_cast<double>(Hs[4]) * nx + static_cast<double>(Hs[5]) * ny +
+                        static_cast<double>(Hs[2]) * nz);
+    }
+    if (t < cnt) {
+      const int o = perm[s0 + t];
+      if (has_text) {
+        const double n3[3] = {nx, ny, nz};
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int r = 0; r < 3; ++r)
+          for (int a = 0; a < 3; ++a) v[r] += static_cast<double>(text.p[3 * a + r][o]) * n3[a];
+        F0 += v[0];
+        F1 += v[1];
+        F2 += v[2];
+      }
+      fx[o] = static_cast<float>(F0);
+      fy[o] = static_cast<float>(F1);
+      fz[o] = static_cast<float>(F2);
+    }
+  }
+}
+
+// P2P, force: F[perm] += (sum over the leaf's near ranges of B[m_j]) . m_i.
+__global__ __launch_bounds__(64) void p2p_force_kernel(
+    int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm, const int2 *__restrict__ rng,
+    const int2 *__restrict__ nr, int nslot, HessConsts hc, float *__restrict__ fx,
+    float *__restrict__ fy, float *__restrict__ fz) {
   __shared__ float4 s_tab[12];
   if (threadIdx.x < 12) {
     const float *row = &hc.curve[threadIdx.x / 3][(threadIdx.x % 3) * 4];
@@ -661,39 +820,20 @@ __global__ __launch_bounds__(64) void leaf_force_kernel(
   const float *tab = reinterpret_cast<const float *>(s_tab);
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
-  const I3 c = demorton(keys[b]);
-  const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
-  const float inv_s = static_cast<float>(1.0 / s);
-  const double far_scale = -kMu0 / (4.0 * kPi) / (s * s * s);
   const int s0 = start[b], cnt = count[b];
-  const float *__restrict__ L = Lt + static_cast<int64_t>(leaf_base + b) * KP;
-  const int32_t *__restrict__ nl = near + static_cast<int64_t>(b) * kNearH;
+  const int nrange = nr[b].x;
+  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double sfac = 3.0 * kMu0 / (4.0 * kPi);
   for (int t0 = 0; t0 < cnt; t0 += 64) {
     const int t = t0 + lane;
-    const bool valid = t < cnt;
-    const SrcRec me = rec[s0 + (valid ? t : cnt - 1)];
+    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
     const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
-    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-    if (use_far) {
-      float Hs[6];
-      l2p_hess<P>(L, static_cast<float>(tx - cx) * inv_s, static_cast<float>(ty - cy) * inv_s,
-                  static_cast<float>(tz - cz) * inv_s, Hs);
-      // T = far_scale * Hess; F = T n
-      F0 = far_scale * (static_cast<double>(Hs[0]) * nx + static_cast<double>(Hs[3]) * ny +
-                        static_cast<double>(Hs[4]) * nz);
-      F1 = far_scale * (static_cast<double>(Hs[3]) * nx + static_cast<double>(Hs[1]) * ny +
-                        static_cast<double>(Hs[5]) * nz);
-      F2 = far_scale * (static_cast<double>(Hs[4]) * nx + static_cast<double>(Hs[5]) * ny +
-                        static_cast<double>(Hs[2]) * nz);
-    }
     double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
-    for (int k = 0; k < kNearH; ++k) {
-      const int sb = nl[k];
-      if (sb < 0) continue;
-      const int j0 = start[sb], j1 = j0 + count[sb];
+    for (int r = 0; r < nrange; ++r) {
+      const int2 q = rg[r];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
-      for (int j = j0; j < j1; ++j) {
+#pragma unroll 2
+      for (int j = q.x; j < q.y; ++j) {
         const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
         const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
         const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
@@ -724,23 +864,14 @@ __global__ __launch_bounds__(64) void leaf_force_kernel(
       am2 += mz0;
       an += n0;
     }
-    F0 += sfac * (ad0 + am0 + static_cast<double>(nx) * an);
-    F1 += sfac * (ad1 + am1 + static_cast<double>(ny) * an);
-    F2 += sfac * (ad2 + am2 + static_cast<double>(nz) * an);
-    if (valid) {
+    if (t < cnt) {
       const int o = perm[s0 + t];
-      if (has_text) {
-        const double n3[3] = {nx, ny, nz};
-        double v[3] = {0.0, 0.0, 0.0};
-        for (int r = 0; r < 3; ++r)
-          for (int a = 0; a < 3; ++a) v[r] += static_cast<double>(text.p[3 * a + r][o]) * n3[a];
-        F0 += v[0];
-        F1 += v[1];
-        F2 += v[2];
-      }
-      fx[o] = static_cast<float>(F0);
-      fy[o] = static_cast<float>(F1);
-      fz[o] = static_cast<float>(F2);
+      fx[o] = static_cast<float>(static_cast<double>(fx[o]) +
+                                 sfac * (ad0 + am0 + static_cast<double>(nx) * an));
+      fy[o] = static_cast<float>(static_cast<double>(fy[o]) +
+                                 sfac * (ad1 + am1 + static_cast<double>(ny) * an));
+      fz[o] = static_cast<float>(static_cast<double>(fz[o]) +
+                                 sfac * (ad2 + am2 + static_cast<double>(nz) * an));
     }
   }
 }
@@ -777,11 +908,6 @@ __global__ void count_m2l_kernel(int nbox, int base, const int32_t *__restrict__
   unsigned long long s = 0;
   for (int k = 0; k < nslot; ++k) s += src[static_cast<int64_t>(base + b) * nslot + k] >= 0 ? 1 : 0;
   atomicAdd(total, s);
-}
-
-__global__ void fill_zero_kernel(int64_t n, float *__restrict__ a) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0.f;
 }
 
 // ---------------------------------------------------------------------------
@@ -879,6 +1005,8 @@ struct FmmState {
   DeviceBuffer box_count;    // int32 [nbox]: particles (leaf) / children (inner)
   DeviceBuffer box_parent;   // int32 [nbox] global parent index
   DeviceBuffer nearF, nearH; // leaf lists
+  DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
+  DeviceBuffer nrF, nrH;     // int2 range counts per leaf (class 0, class 1)
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
@@ -1147,6 +1275,18 @@ Status fmm_build(mf_plan *plan) {
                      dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells,
                      tab.nearH.as<int3>(), kNearH, F.nearH.as<int32_t>());
   MF_LAUNCH_CHECK();
+  MF_TRY(need<int2>(F.rngF, static_cast<int64_t>(nleaf) * kNearF));
+  MF_TRY(need<int2>(F.rngH, static_cast<int64_t>(nleaf) * kNearH));
+  MF_TRY(need<int2>(F.nrF, nleaf));
+  MF_TRY(need<int2>(F.nrH, nleaf));
+  hipLaunchKernelGGL(near_ranges_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                     F.nearF.as<int32_t>(), kNearF, kNearFMixed, F.box_first.as<int32_t>(),
+                     F.box_count.as<int32_t>(), F.rngF.as<int2>(), F.nrF.as<int2>());
+  MF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(near_ranges_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                     F.nearH.as<int32_t>(), kNearH, kNearH, F.box_first.as<int32_t>(),
+                     F.box_count.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>());
+  MF_LAUNCH_CHECK();
   // 5. M2L source slots (levels >= 2) and the class-sorted tile lists
   MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox) * tab.nslotF));
   MF_TRY(need<int32_t>(F.srcH, static_cast<int64_t>(nleaf) * tab.nslotH));
@@ -1194,8 +1334,8 @@ Status fmm_build(mf_plan *plan) {
       if (l == D) F.leaf_tile_start = static_cast<int>(th.size());
       for (int par = 0; par < 8; ++par) {
         const int c = cc[l * 8 + par];
-        for (int s = 0; s < c; s += 64) {
-          M2LTile t{pos + s, std::min(64, c - s), par, tab.nslotF};
+        for (int s = 0; s < c; s += kTileT) {
+          M2LTile t{pos + s, std::min(kTileT, c - s), par, tab.nslotF};
           tf.push_back(t);
           M2LTile u = t;
           if (l == D) {
@@ -1302,9 +1442,9 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
       // F lists for levels < D; at the leaf level the force uses the H lists,
       // whose source slots live in srcH (indexed by leaf = global index).
       if (!hess) {
-        hipLaunchKernelGGL((m2l_kernel<KP>), dim3(ntiles), dim3(2 * KP), 0, ctx->stream,
-                           F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                           F.tilesF.as<M2LTile>(), F.class_ids2.as<int32_t>(),
+        hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((ntiles + 7) / 8)), dim3(2 * KP), 0,
+                           ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
+                           F.tilesF.as<M2LTile>(), ntiles, F.class_ids2.as<int32_t>(),
                            F.srcF.as<int32_t>(), tab.nslotF, tab.slot_tab.as<int32_t>(), tab.slot_ld);
         MF_LAUNCH_CHECK();
       } else {
@@ -1312,17 +1452,18 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
         // by level, so the leaf tiles are the trailing ones.
         const int first_leaf_tile = F.leaf_tile_start;
         if (first_leaf_tile > 0) {
-          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(first_leaf_tile), dim3(2 * KP), 0, ctx->stream,
-                             F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                             F.tilesH.as<M2LTile>(), F.class_ids2.as<int32_t>(),
+          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((first_leaf_tile + 7) / 8)), dim3(2 * KP),
+                             0, ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
+                             F.tilesH.as<M2LTile>(), first_leaf_tile, F.class_ids2.as<int32_t>(),
                              F.srcF.as<int32_t>(), tab.nslotF, tab.slot_tab.as<int32_t>(),
                              tab.slot_ld);
           MF_LAUNCH_CHECK();
         }
         if (ntiles - first_leaf_tile > 0) {
-          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(ntiles - first_leaf_tile), dim3(2 * KP), 0,
+          const int nl = ntiles - first_leaf_tile;
+          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((nl + 7) / 8)), dim3(2 * KP), 0,
                              ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                             F.tilesH.as<M2LTile>() + first_leaf_tile, F.class_ids2.as<int32_t>(),
+                             F.tilesH.as<M2LTile>() + first_leaf_tile, nl, F.class_ids2.as<int32_t>(),
                              F.srcH.as<int32_t>(), tab.nslotH, tab.slot_tab.as<int32_t>(),
                              tab.slot_ld);
           MF_LAUNCH_CHECK();
@@ -1362,13 +1503,23 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   MF_LAUNCH_CHECK();
   MF_TRY((upward_and_m2l<P, KP>(plan, false)));
   const int D = F.D, nleaf = F.count[D];
+  const float h0 = hconst ? hconst[0] : 0.f, h1 = hconst ? hconst[1] : 0.f,
+              h2 = hconst ? hconst[2] : 0.f;
+  const int use_far = D >= 2 ? 1 : 0;
+  if (use_far) {
+    KernelTimer timer(ctx, "l2p_field");
+    hipLaunchKernelGGL((l2p_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                       F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
+                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                       F.Lt.as<float>(), F.lo[0], F.lo[1], F.lo[2], F.s_leaf, h0, h1, h2, ox, oy,
+                       oz);
+    MF_LAUNCH_CHECK();
+  }
   KernelTimer timer(ctx, "p2p_field");
-  hipLaunchKernelGGL((leaf_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.box_keys.as<uint64_t>(),
-                     F.rec.as<SrcRec>(), F.perm.as<int32_t>(), F.nearF.as<int32_t>(),
-                     F.Lt.as<float>(), 0, D >= 2 ? 1 : 0, F.lo[0], F.lo[1], F.lo[2], F.s_leaf,
-                     plan->fc, hconst ? hconst[0] : 0.f, hconst ? hconst[1] : 0.f,
-                     hconst ? hconst[2] : 0.f, ox, oy, oz);
+  hipLaunchKernelGGL(p2p_field_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.rec.as<SrcRec>(),
+                     F.perm.as<int32_t>(), F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, use_far,
+                     plan->fc, h0, h1, h2, ox, oy, oz);
   MF_LAUNCH_CHECK();
   return Status::Ok();
 }
@@ -1394,12 +1545,20 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
     }
   }
   const int D = F.D, nleaf = F.count[D];
+  {
+    KernelTimer timer(ctx, "l2p_force");
+    hipLaunchKernelGGL((l2p_force_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                       F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
+                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                       F.Lt.as<float>(), D >= 2 ? 1 : 0, F.lo[0], F.lo[1], F.lo[2], F.s_leaf, t,
+                       has_text, fx, fy, fz);
+    MF_LAUNCH_CHECK();
+  }
   KernelTimer timer(ctx, "p2p_force");
-  hipLaunchKernelGGL((leaf_force_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.box_keys.as<uint64_t>(),
-                     F.rec.as<SrcRec>(), F.perm.as<int32_t>(), F.nearH.as<int32_t>(),
-                     F.Lt.as<float>(), 0, D >= 2 ? 1 : 0, F.lo[0], F.lo[1], F.lo[2], F.s_leaf,
-                     plan->hc, t, has_text, fx, fy, fz);
+  hipLaunchKernelGGL(p2p_force_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.rec.as<SrcRec>(),
+                     F.perm.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc,
+                     fx, fy, fz);
   MF_LAUNCH_CHECK();
   return Status::Ok();
 }

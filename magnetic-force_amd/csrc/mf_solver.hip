@@ -286,11 +286,20 @@ Status read_scalar(mf_ctx *ctx, const double *d, double *h) {
 }
 
 // r = b - A x with A x = x - G(kappa x); also writes p = r if p != null.
-Status residual(mf_plan *plan, double kappa, CV3 b, V3 x, const Work &w, bool set_p) {
+Status residual(mf_plan *plan, double kappa, CV3 b, V3 x, const Work &w, bool set_p,
+                bool zero_x = false) {
   mf_ctx *ctx = plan->ctx;
   const int64_t n = plan->n;
   float *g = w.g;
-  MF_TRY(plan_apply_G(plan, static_cast<float>(kappa), x.x, x.y, x.z, nullptr, g, g + n, g + 2 * n));
+  if (zero_x) {
+    // x = 0: G(kappa x) = 0 and r = b, no product needed
+    MF_TRY(vec_fill(ctx, n, 0.f, x.x));
+    MF_TRY(vec_fill(ctx, n, 0.f, x.y));
+    MF_TRY(vec_fill(ctx, n, 0.f, x.z));
+    MF_TRY(vec_fill(ctx, 3 * n, 0.f, g));
+  } else {
+    MF_TRY(plan_apply_G(plan, static_cast<float>(kappa), x.x, x.y, x.z, nullptr, g, g + n, g + 2 * n));
+  }
   V3 r = v3(w.r, n);
   V3 p = set_p ? v3(w.p, n) : V3{nullptr, nullptr, nullptr};
   hipLaunchKernelGGL(cg_init_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, b,
@@ -307,8 +316,8 @@ Status solve_cg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_opts &o
   MF_TRY(get_work(ctx, n, w));
   const int nb = red_blocks(n);
   MF_TRY(vec_dot_device(ctx, n, b.x, b.y, b.z, b.x, b.y, b.z, w.sc + S_BB));
-  MF_TRY(residual(plan, kappa, b, x, w, true));
-  st->matvecs += 1;
+  MF_TRY(residual(plan, kappa, b, x, w, true, o.zero_start != 0));
+  if (!o.zero_start) st->matvecs += 1;
   MF_TRY(vec_dot_device(ctx, n, w.r, w.r + n, w.r + 2 * n, w.r, w.r + n, w.r + 2 * n, w.sc + S_RR));
   double bb = 0.0, rr = 0.0;
   MF_TRY(read_scalar(ctx, w.sc + S_BB, &bb));
@@ -386,9 +395,12 @@ Status solve_gmres(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_opts
     return Status::Ok();
   };
 
+  bool first_cycle = true;
   while (true) {
-    MF_TRY(residual(plan, kappa, b, x, w, false));
-    st->matvecs += 1;
+    const bool zero_x = first_cycle && o.zero_start != 0;
+    first_cycle = false;
+    MF_TRY(residual(plan, kappa, b, x, w, false, zero_x));
+    if (!zero_x) st->matvecs += 1;
     double rr = 0.0;
     MF_TRY(vec_dot(ctx, n, w.r, w.r + n, w.r + 2 * n, w.r, w.r + n, w.r + 2 * n, &rr));
     resid = std::sqrt(rr);
@@ -589,6 +601,11 @@ Status solve(mf_plan *plan, double kappa, const float *bx, const float *by, cons
       MF_TRY(solve_gmres(plan, kappa, b, x, o, st));
       break;
     case MF_SOLVE_REF_NLCG:
+      if (o.zero_start) {
+        MF_TRY(vec_fill(plan->ctx, plan->n, 0.f, xx));
+        MF_TRY(vec_fill(plan->ctx, plan->n, 0.f, xy));
+        MF_TRY(vec_fill(plan->ctx, plan->n, 0.f, xz));
+      }
       MF_TRY(solve_ref_nlcg(plan, kappa, b, x, o, st));
       break;
     default:
@@ -666,9 +683,9 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
     hipLaunchKernelGGL(f64_to_f32_scaled_kernel, dim3(grid_for(n3, 256)), dim3(256), 0,
                        ctx->stream, n3, 1.0 / rn, r64, rhs);
     MF_LAUNCH_CHECK();
-    MF_HIP(hipMemsetAsync(d, 0, sizeof(float) * n3, ctx->stream));
     mf_solve_opts inner{};
     inner.method = MF_SOLVE_CG;
+    inner.zero_start = 1;
     inner.maxit = opts.maxit > 0 ? opts.maxit : 500;
     inner.rtol = 1e-5;
     mf_solve_stats ist{};

@@ -68,7 +68,8 @@ class PlanInfo(ctypes.Structure):
 class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("maxit", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("refine_fp64", ctypes.c_int),
-                ("true_residual", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
+                ("true_residual", ctypes.c_int), ("zero_start", ctypes.c_int),
+                ("reserved", ctypes.c_int * 1)]
 
 
 class SolveStats(ctypes.Structure):
@@ -405,9 +406,11 @@ class Plan:
               maxit: int = 1000, restart: int = 30, true_residual: bool = False):
         import torch
         bb = _soa(b, self.n)
-        x = torch.zeros_like(bb) if x0 is None else _soa(x0, self.n).clone()
+        # x0 None: the solver starts from zero itself, saving one product
+        x = torch.empty_like(bb) if x0 is None else _soa(x0, self.n).clone()
         opts = SolveOpts(method=method, restart=restart, maxit=maxit, rtol=rtol,
-                         true_residual=1 if true_residual else 0)
+                         true_residual=1 if true_residual else 0,
+                         zero_start=1 if x0 is None else 0)
         st = SolveStats()
         _check(lib().mf_plan_solve(self.h_plan, float(kappa), _ptr(bb, 0), _ptr(bb, 1),
                                    _ptr(bb, 2), _ptr(x, 0), _ptr(x, 1), _ptr(x, 2),
