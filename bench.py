@@ -28,7 +28,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "magnetic-force_amd"))
 
-FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector peak
+FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector / FP32 MFMA peak
+BF16_PEAK_TFLOPS = 2500.0       # MI355X published dense BF16 MFMA peak
 FIELD_FLOPS_PER_PAIR = 32       # far-branch field pair, FMA = 2, rsqrt = 1
 FORCE_FLOPS_PER_PAIR = 64       # far-branch 6-component force tensor pair
 
@@ -51,6 +52,10 @@ def parse():
     p.add_argument("--workload", default="pool1m")
     p.add_argument("--mode", default="auto", choices=["auto", "fmm", "allpairs"])
     p.add_argument("--rtol", type=float, default=1e-6)
+    p.add_argument("--m2l", default="bf16x3", choices=["fp32", "bf16x6", "bf16x3"],
+                   help="FMM M2L precision (bf16x3: the mixed-precision FMM)")
+    p.add_argument("--no-compare", action="store_true",
+                   help="skip the FP32-M2L FMM run of the same workload")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--check-samples", type=int, default=4096)
     p.add_argument("--no-cpu", action="store_true")
@@ -70,13 +75,14 @@ def parse():
 class Stepper:
     """Runs plan build + solve + force for one particle system."""
 
-    def __init__(self, ctx, system, mode, rtol):
+    def __init__(self, ctx, system, mode, rtol, m2l=0):
         import torch
         import magforce
         self.ctx = ctx
         self.s = system
         self.mode = mode
         self.rtol = rtol
+        self.m2l = m2l
         self.pos = to_dev(system.pos)
         self.b = magforce.uniform_field(system.n, system.hconst)
         self.x = torch.zeros_like(self.b)
@@ -87,7 +93,8 @@ class Stepper:
     def step(self):
         import magforce
         if self.plan is None:
-            self.plan = magforce.Plan(self.ctx, self.pos, self.s.h, mode=self.mode)
+            self.plan = magforce.Plan(self.ctx, self.pos, self.s.h, mode=self.mode,
+                                      m2l_precision=self.m2l)
         else:
             self.plan.update_positions(self.pos)
         res = self.plan.solve(self.s.kappa, self.b, x0=None, rtol=self.rtol)
@@ -128,6 +135,27 @@ def measure_allpairs(ctx, log_prefix="allpairs"):
     log(f"{log_prefix}: {res}")
     plan.close()
     return res
+
+
+def time_fp32_fmm(ctx, system, args):
+    """The same step with FP32 M2L (the plain FP32 FMM build), same steps."""
+    import torch
+    import magforce
+    st = Stepper(ctx, system, magforce.MODE_FMM, args.rtol, magforce.M2L_FP32)
+    for _ in range(max(args.warmup, 1)):
+        st.step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        st.step()
+    e1.record()
+    torch.cuda.synchronize()
+    out = {"ms_per_step": e0.elapsed_time(e1) / args.steps, "m2l": "fp32",
+           "solver": st.stats}
+    st.plan.close()
+    log(f"fp32 fmm: {out}")
+    return out
 
 
 def check_error(ctx, st: Stepper, samples: int):
@@ -224,12 +252,13 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: {nbytes} bytes read + {nbytes} bytes written")
         del a, b
-    st = Stepper(ctx, system, mode, args.rtol)
+    m2l_code = {"fp32": magforce.M2L_FP32, "bf16x6": magforce.M2L_BF16X6,
+                "bf16x3": magforce.M2L_BF16X3}[args.m2l]
+    st = Stepper(ctx, system, mode, args.rtol, m2l_code)
     for w in range(args.warmup):
         st.step()
         torch.cuda.synchronize()
         log(f"warmup {w} done: {st.stats}")
-    dom = "p2p_field" if mode == magforce.MODE_FMM else "field"
     ctx.reset_timing()
     ctx.enable_timing(True)
     torch.cuda.synchronize()
@@ -243,8 +272,9 @@ def main():
     wall = time.perf_counter() - t0
     ms_step = e0.elapsed_time(e1) / args.steps
     log(f"timed: {args.steps} steps, {ms_step:.2f} ms/step (wall {wall / args.steps * 1e3:.2f})")
-    kms, kcnt = ctx.kernel_time(dom)
-    fms, fcnt = ctx.kernel_time("force")
+    ktimes = {k: ctx.kernel_time(k) for k in
+              ("field", "force", "p2m", "m2m", "m2l", "m2l_force", "l2l", "l2p_field",
+               "p2p_field", "l2p_force", "p2p_force")}
     ctx.enable_timing(False)
     matvecs = st.stats["matvecs"]
     if args.profile_only:
@@ -252,30 +282,60 @@ def main():
         return
 
     n = system.n
-    avg_ms = kms / max(kcnt, 1)
     info = st.plan.info()
-    # algorithmic FLOPs of one launch of the dominant kernel: direct pairs x 32
-    flops = float(info["p2p_field_pairs"]) * FIELD_FLOPS_PER_PAIR
-    achieved = flops / (avg_ms * 1e-3) / 1e12
-    traffic = None
-    if args.traffic and os.path.exists(args.traffic):
-        with open(args.traffic) as f:
-            t = json.load(f)
-        if t.get("workload") == system.name and t.get("kernel") == dom:
-            traffic = t["hbm_bytes_per_launch"]
-    roofline = {"kernel": dom, "bound": "fp32-valu", "achieved": achieved,
-                "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
-                "traffic": traffic, "avg_launch_ms": avg_ms, "launches": kcnt,
+
+    def kernel_roofline(name, flops, peak, bound):
+        ms, cnt = ktimes[name]
+        avg = ms / max(cnt, 1)
+        achieved = flops / (avg * 1e-3) / 1e12 if avg > 0 else 0.0
+        return {"kernel": name, "bound": bound, "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "avg_launch_ms": avg, "launches": cnt, "ms_per_step": ms / args.steps,
                 "flops_per_launch": flops}
+
     if mode == magforce.MODE_FMM:
+        # P2P: direct pairs x 32 FLOPs against the FP32 vector peak.  M2L:
+        # 2 (p+1)^4 FLOPs per box pair (dense (p+1)^2 x (p+1)^2 real operator,
+        # padding excluded) against the MFMA peak of the instruction used; the
+        # BF16-split kernels issue 6 (or 3) BF16 products per FP32 product, so
+        # their FP32-equivalent peak is the BF16 peak / 6 (or / 3).
+        order = info["order"]
+        m2l_flops = float(info["m2l_field_pairs"]) * 2.0 * (order + 1) ** 4
+        m2l_peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x6": BF16_PEAK_TFLOPS / 6,
+                    "bf16x3": BF16_PEAK_TFLOPS / 3}[args.m2l]
+        cands = [kernel_roofline("p2p_field", float(info["p2p_field_pairs"]) * FIELD_FLOPS_PER_PAIR,
+                                 FP32_PEAK_TFLOPS, "valu"),
+                 kernel_roofline("m2l", m2l_flops, m2l_peak, "mfma")]
+        cands.sort(key=lambda r: -r["ms_per_step"])
+        roofline = cands[0]
+        roofline["others"] = cands[1:]
         breakdown = {}
-        for k in ("p2m", "m2m", "m2l", "l2l", "l2p_field", "p2p_field", "l2p_force", "p2p_force"):
-            ms, cnt = ctx.kernel_time(k)
+        for k in ("p2m", "m2m", "m2l", "m2l_force", "l2l", "l2p_field", "p2p_field", "l2p_force",
+                  "p2p_force"):
+            ms, cnt = ktimes[k]
             breakdown[k] = {"ms_total": ms / args.steps, "launches": cnt // max(args.steps, 1)}
         roofline["per_step_breakdown"] = breakdown
         roofline["plan"] = info
+    else:
+        roofline = kernel_roofline("field", float(n) * n * FIELD_FLOPS_PER_PAIR, FP32_PEAK_TFLOPS,
+                                   "valu")
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            t = json.load(f)
+        if t.get("workload") == system.name and t.get("kernel") == roofline["kernel"]:
+            roofline["traffic"] = t["hbm_bytes_per_launch"]
 
+    compare = None
+    if mode == magforce.MODE_FMM and args.m2l != "fp32" and not args.no_compare:
+        compare = time_fp32_fmm(ctx, system, args)
+        compare["speedup"] = compare["ms_per_step"] / ms_step
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)
+    if allpairs is not None:
+        # plain FP32 all-pairs step at this n, from the measured pair rates
+        # (force tensor pairs cost 2x field pairs): an estimate, not a run
+        per_field_ms = float(n) * n / (allpairs["gpairs_per_s"] * 1e9) * 1e3
+        allpairs["est_step_ms_at_workload_n"] = per_field_ms * (matvecs + 2)
+        allpairs["est_speedup_of_value"] = allpairs["est_step_ms_at_workload_n"] / ms_step
     check = None if args.no_check else check_error(ctx, st, args.check_samples)
     cpu = None if args.no_cpu else cpu_baseline(system, args.cpu_seconds, matvecs)
 
@@ -285,10 +345,13 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{system.name} (n={n}, h={system.h}, chi={system.chi}, "
-                               f"H=(0,0,1), CG rtol {args.rtol}, {mode_name})",
+                               f"H=(0,0,1), CG rtol {args.rtol}, {mode_name}"
+                               + (f", M2L {args.m2l})" if mode == magforce.MODE_FMM else ")"),
                    "mode": mode_name, "n": n, "matvecs_per_step": matvecs,
+                   "m2l_precision": args.m2l if mode == magforce.MODE_FMM else None,
                    "solver": st.stats},
         "error": check,
+        "fp32_fmm": compare,
         "allpairs": allpairs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -49,6 +49,14 @@ typedef enum mf_precision {
   MF_PREC_FP64 = 1       /* FP64 throughout (the GPU FP64 check path) */
 } mf_precision;
 
+typedef enum mf_m2l_precision {
+  MF_M2L_FP32 = 0,   /* FP32 MFMA (v_mfma_f32_16x16x4_f32) */
+  MF_M2L_BF16X6 = 1, /* BF16 MFMA, operands split in 3 BF16 parts, 6 products,
+                        FP32 accumulation: FP32-level accuracy */
+  MF_M2L_BF16X3 = 2  /* BF16 MFMA, 2 parts, 3 products: ~1e-5 relative per
+                        translation (inner products of the mixed solve) */
+} mf_m2l_precision;
+
 typedef enum mf_mode {
   MF_MODE_ALLPAIRS = 0,  /* reference "naive" mode, magnetic_force.cu:1200-1222 */
   MF_MODE_FMM = 1        /* reference default FMM mode, magnetic_force.cu:1223-1270 */
@@ -147,9 +155,9 @@ int mf_force_subset_f64(mf_ctx *ctx, int64_t n, const double *x, const double *y
 /* ---- plans: one particle set, reused across products and solves ---------- */
 typedef struct mf_plan_opts {
   int mode;          /* mf_mode */
-  int leaf_size;     /* FMM: target particles per leaf (0 = autotune) */
-  int order;         /* FMM: expansion order p, degrees 0..p-1 (0 = default) */
-  int m2l_precision; /* FMM: 0 = FP32, 1 = BF16 operands with FP32 accumulation */
+  int leaf_size;     /* unused (the octree leaf side is fixed at 2h); must be 0 */
+  int order;         /* FMM: expansion order p (degrees 0..p): 6, 8 or 10; 0 = 10 */
+  int m2l_precision; /* FMM: mf_m2l_precision of mf_plan_field / mf_plan_force */
   int reserved[4];
 } mf_plan_opts;
 

@@ -442,6 +442,10 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown plan mode");
     }
+    if (plan->opts.m2l_precision < MF_M2L_FP32 || plan->opts.m2l_precision > MF_M2L_BF16X3) {
+      delete plan;
+      return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
+    }
     plan->h = h;
     plan->fc = mf::make_field_consts(h);
     plan->hc = mf::make_hess_consts(h);

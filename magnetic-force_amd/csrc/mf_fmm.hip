@@ -21,10 +21,11 @@
 //    of the parent's N-neighbours minus the near set.
 //  * Expansions in scaled real dofs (mf_expansion.hpp): M2M/L2L are 8 fixed
 //    matrices, M2L one fixed matrix per offset (column-major tables in HBM).
-//  * M2L runs on MFMA (v_mfma_f32_16x16x4_f32): a workgroup owns 64 target
-//    boxes of one (level, parity) class and walks that class's offset list;
-//    per offset it gathers the 64 source multipoles into LDS and multiplies by
-//    the offset's matrix, accumulating the 64 local expansions in registers.
+//  * M2L runs on MFMA (FP32, or BF16 with split operands): a workgroup owns
+//    64 target boxes of one (level, parity) class and walks that class's
+//    offset list; per offset it gathers the 64 source multipoles into LDS and
+//    multiplies by the offset's matrix, accumulating the 64 local expansions
+//    in registers.
 //  * Leaf pass: L2P (one wave per leaf, local expansion through scalar loads)
 //    writes H_const + far field in the caller's particle order; the near-field
 //    kernel then adds the sums over the leaf's near boxes, whose particle
@@ -34,6 +35,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -367,18 +369,41 @@ __global__ __launch_bounds__(64) void l2l_kernel(int nchild, int child_base,
 }
 
 // ---------------------------------------------------------------------------
-// M2L on MFMA.  A workgroup owns a tile of up to kTileT = 128 target boxes of
+// M2L on MFMA.  A workgroup owns a tile of up to kTileT = 64 target boxes of
 // one (level, parity) class and walks that class's offset list.  Per offset
 // it gathers the tile's source multipoles into LDS (skipping the offset when
 // no target of the tile has a source there) and multiplies them by the
 // offset's KP x KP matrix, accumulating the local expansions in registers.
-// KP/32 waves: wave w owns rows [32 w, 32 w + 32) of all 64 columns:
-// 2 x 4 blocks of 16x16, K stepped by 4.
-// MFMA 16x16x4 f32 operand layout (wave64): lane l supplies A[l % 16][l / 16]
-// and B[l / 16][l % 16]; D[4 (l / 16) + r][l % 16] = acc[r].
-// Workgroups are remapped so that each XCD (round-robin dispatch) walks a
-// contiguous range of tiles: neighbouring tiles share classes, offset
-// matrices and source multipoles, which then stay in that XCD's L2.
+// KP/32 waves: wave w owns rows [32 w, 32 w + 32) of all 64 columns, as
+// 2 x 4 blocks of 16x16.  Workgroups are remapped so that each XCD
+// (round-robin dispatch) walks a contiguous range of tiles: neighbouring tiles
+// share classes, offset matrices and source multipoles, which then stay in
+// that XCD's L2.
+//
+// Software pipeline: the gather of the next offset runs under the current
+// offset's MFMAs.  The iteration for offset c stages the multipoles of c
+// (held in registers) into LDS, issues the multipole loads of the next offset
+// n1 (whose source indices were loaded one iteration earlier) and the
+// source-index loads of n2, then multiplies offset c from LDS.  Each thread
+// holds 8 float4 of the 64 x KP block: element e = tid + it * NT is target
+// e / (KP / 4), dofs 4 (e % (KP / 4)) .. + 3.
+//
+// Precisions (NPART):
+//  * 0: FP32 MFMA v_mfma_f32_16x16x4_f32, K stepped by 4.  Operand layout
+//    (wave64): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
+//    D[4 (l / 16) + r][l % 16] = acc[r].  Offset matrices column-major.
+//  * 2 or 3: BF16 MFMA v_mfma_f32_16x16x32_bf16, K stepped by 32; lane l
+//    supplies A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16].
+//    Every FP32 operand x is split into BF16 parts x = x0 + x1 + x2 (each the
+//    round-to-nearest-even BF16 of the remainder; three parts hold all 24
+//    significand bits).  NPART = 3 sums the six products a_i b_j with
+//    i + j <= 2 (~2^-24 relative per product, FP32 level); NPART = 2 the
+//    three with i + j <= 1 (~2^-16).  Accumulation is FP32 in the MFMA.  A
+//    BF16 MFMA does 16x the multiply-adds of the FP32 one per cycle, so six
+//    products cost 3/8 of the FP32 MFMA time.  The offset matrices are stored
+//    pre-split (from their FP64 values), row-major, so that a lane's eight
+//    consecutive k of one row are one 16-byte load; the gathered multipoles
+//    are split as they are staged into LDS.
 constexpr int kTileT = 64;
 
 struct M2LTile {
@@ -388,19 +413,74 @@ struct M2LTile {
   int nslot;
 };
 
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint32_t bf16_bits_rne(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+template <int NPART>
+__device__ __forceinline__ void split_bf16(float x, uint32_t (&p)[NPART]) {
+  float r = x;
+#pragma unroll
+  for (int i = 0; i < NPART; ++i) {
+    p[i] = bf16_bits_rne(r);
+    if (i + 1 < NPART) r -= __uint_as_float(p[i] << 16);
+  }
+}
+
+constexpr int kGatherIt = kTileT / 8;  // 64 * KP / 4 / (2 KP)
+
+__device__ __forceinline__ int next_slot(const int32_t *__restrict__ smap, int s, int nslot) {
+  while (s < nslot && smap[s] < 0) ++s;
+  return s;
+}
+
 template <int KP>
+__device__ __forceinline__ void m2l_src_load(const int *s_tgt, const int32_t *__restrict__ src_slots,
+                                             int src_ld, int slot, int tid,
+                                             int (&src)[kGatherIt]) {
+#pragma unroll
+  for (int it = 0; it < kGatherIt; ++it) {
+    const int tgt = s_tgt[(tid + it * 2 * KP) / (KP / 4)];
+    src[it] = tgt >= 0 ? src_slots[static_cast<int64_t>(tgt) * src_ld + slot] : -1;
+  }
+}
+
+template <int KP>
+__device__ __forceinline__ int m2l_mult_load(const float *__restrict__ Mt,
+                                             const int (&src)[kGatherIt], int tid,
+                                             float4 (&v)[kGatherIt]) {
+  int any = 0;
+#pragma unroll
+  for (int it = 0; it < kGatherIt; ++it) {
+    const int q = (tid + it * 2 * KP) % (KP / 4);
+    v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (src[it] >= 0) {
+      v[it] = reinterpret_cast<const float4 *>(Mt + static_cast<int64_t>(src[it]) * KP)[q];
+      any = 1;
+    }
+  }
+  return any;
+}
+
+template <int KP, int NPART>
 __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ Mt,
-                                                     float *__restrict__ Lt,
-                                                     const float *__restrict__ T,
-                                                     const M2LTile *__restrict__ tiles,
-                                                     int ntiles,
-                                                     const int32_t *__restrict__ sorted_boxes,
-                                                     const int32_t *__restrict__ src_slots,
-                                                     int src_ld,
-                                                     const int32_t *__restrict__ slot_table,
-                                                     int slot_ld) {
-  constexpr int NT = 2 * KP;  // threads: KP/32 waves
-  __shared__ float sB[kTileT][KP + 4];
+                                                          float *__restrict__ Lt,
+                                                          const void *__restrict__ Tv,
+                                                          const M2LTile *__restrict__ tiles,
+                                                          int ntiles,
+                                                          const int32_t *__restrict__ sorted_boxes,
+                                                          const int32_t *__restrict__ src_slots,
+                                                          int src_ld,
+                                                          const int32_t *__restrict__ slot_table,
+                                                          int slot_ld) {
+  constexpr int NT = 2 * KP;
+  constexpr int LDF = KP + 4;  // FP32 row pitch (floats)
+  constexpr int LDB = KP + 8;  // BF16 row pitch (elements)
+  constexpr int kLds = NPART == 0 ? kTileT * LDF * 4 : NPART * kTileT * LDB * 2;
+  __shared__ __attribute__((aligned(16))) unsigned char s_raw[kLds];
   __shared__ int s_tgt[kTileT];
   const int per_xcd = (ntiles + 7) / 8;
   const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
@@ -408,7 +488,6 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
   const M2LTile tile = tiles[tile_id];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int wr = wave;
   for (int j = tid; j < kTileT; j += NT) s_tgt[j] = j < tile.count ? sorted_boxes[tile.first + j] : -1;
   __syncthreads();
   typedef float f4 __attribute__((ext_vector_type(4)));
@@ -418,55 +497,112 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f4{0.f, 0.f, 0.f, 0.f};
   const int32_t *__restrict__ smap = slot_table + static_cast<int64_t>(tile.slots) * slot_ld;
-  const int row0 = 32 * wr + (lane & 15);
-  const int col0 = lane & 15;
+  const int r16 = lane & 15;
   const int kq = lane >> 4;
-  for (int slot = 0; slot < tile.nslot; ++slot) {
-    const int tix = smap[slot];
-    if (tix < 0) continue;
-    __syncthreads();  // previous slot's B fully consumed
-    int any = 0;
-    for (int e = tid; e < kTileT * (KP / 4); e += NT) {
+  int cur = next_slot(smap, 0, tile.nslot);
+  int src[kGatherIt];
+  float4 v[kGatherIt];
+  int any = 0;
+  int nxt = tile.nslot;
+  if (cur < tile.nslot) {
+    m2l_src_load<KP>(s_tgt, src_slots, src_ld, cur, tid, src);
+    any = m2l_mult_load<KP>(Mt, src, tid, v);
+    nxt = next_slot(smap, cur + 1, tile.nslot);
+    if (nxt < tile.nslot) m2l_src_load<KP>(s_tgt, src_slots, src_ld, nxt, tid, src);
+  }
+  while (cur < tile.nslot) {
+    __syncthreads();  // the previous offset's operands are consumed
+#pragma unroll
+    for (int it = 0; it < kGatherIt; ++it) {
+      const int e = tid + it * NT;
       const int j = e / (KP / 4), q = e % (KP / 4);
-      const int tgt = s_tgt[j];
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (tgt >= 0) {
-        const int src = src_slots[static_cast<int64_t>(tgt) * src_ld + slot];
-        if (src >= 0) {
-          v = reinterpret_cast<const float4 *>(Mt + static_cast<int64_t>(src) * KP)[q];
-          any = 1;
+      if constexpr (NPART == 0) {
+        float *sF = reinterpret_cast<float *>(s_raw);
+        *reinterpret_cast<float4 *>(&sF[j * LDF + 4 * q]) = v[it];
+      } else {
+        uint16_t *sB = reinterpret_cast<uint16_t *>(s_raw);
+        uint32_t px[NPART], py[NPART], pz[NPART], pw[NPART];
+        split_bf16<NPART>(v[it].x, px);
+        split_bf16<NPART>(v[it].y, py);
+        split_bf16<NPART>(v[it].z, pz);
+        split_bf16<NPART>(v[it].w, pw);
+#pragma unroll
+        for (int p = 0; p < NPART; ++p)
+          *reinterpret_cast<uint2 *>(&sB[(p * kTileT + j) * LDB + 4 * q]) =
+              make_uint2(px[p] | (py[p] << 16), pz[p] | (pw[p] << 16));
+      }
+    }
+    const int go = __syncthreads_or(any);
+    const int tix = smap[cur];
+    const int nn = nxt < tile.nslot ? next_slot(smap, nxt + 1, tile.nslot) : tile.nslot;
+    if (nxt < tile.nslot) {
+      any = m2l_mult_load<KP>(Mt, src, tid, v);
+      if (nn < tile.nslot) m2l_src_load<KP>(s_tgt, src_slots, src_ld, nn, tid, src);
+    }
+    if (go) {
+      if constexpr (NPART == 0) {
+        const float *sF = reinterpret_cast<const float *>(s_raw);
+        const float *__restrict__ A = static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP;
+        const int row0 = 32 * wave + r16;
+#pragma unroll 4
+        for (int k0 = 0; k0 < KP; k0 += 4) {
+          const int k = k0 + kq;
+          const float a0 = A[k * KP + row0];
+          const float a1 = A[k * KP + row0 + 16];
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            const float bv = sF[(r16 + 16 * cb) * LDF + k];
+            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][cb], 0, 0, 0);
+            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][cb], 0, 0, 0);
+          }
+        }
+      } else {
+        const uint16_t *sB = reinterpret_cast<const uint16_t *>(s_raw);
+        const uint16_t *__restrict__ A =
+            static_cast<const uint16_t *>(Tv) + static_cast<int64_t>(tix) * 3 * KP * KP;
+#pragma unroll 1
+        for (int k0 = 0; k0 < KP; k0 += 32) {
+          const int kk = k0 + 8 * kq;
+          bf16x8 a[2][NPART];
+#pragma unroll
+          for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int p = 0; p < NPART; ++p)
+              a[rb][p] = *reinterpret_cast<const bf16x8 *>(
+                  A + (static_cast<int64_t>(p) * KP + 32 * wave + 16 * rb + r16) * KP + kk);
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb) {
+            bf16x8 b[NPART];
+#pragma unroll
+            for (int p = 0; p < NPART; ++p)
+              b[p] = *reinterpret_cast<const bf16x8 *>(&sB[(p * kTileT + 16 * cb + r16) * LDB + kk]);
+#pragma unroll
+            for (int rb = 0; rb < 2; ++rb) {
+              if constexpr (NPART == 3) {
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[2], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[1], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][2], b[0], acc[rb][cb], 0, 0, 0);
+              }
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[1], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[0], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[0], acc[rb][cb], 0, 0, 0);
+            }
+          }
         }
       }
-      sB[j][4 * q] = v.x;
-      sB[j][4 * q + 1] = v.y;
-      sB[j][4 * q + 2] = v.z;
-      sB[j][4 * q + 3] = v.w;
     }
-    if (!__syncthreads_or(any)) continue;  // no source at this offset for the whole tile
-    const float *__restrict__ A = T + static_cast<int64_t>(tix) * KP * KP;
-#pragma unroll 4
-    for (int k0 = 0; k0 < KP; k0 += 4) {
-      const int k = k0 + kq;
-      const float a0 = A[k * KP + row0];
-      const float a1 = A[k * KP + row0 + 16];
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const float bv = sB[col0 + 16 * cb][k];
-        acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][cb], 0, 0, 0);
-        acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][cb], 0, 0, 0);
-      }
-    }
+    cur = nxt;
+    nxt = nn;
   }
-  // write the local expansions (every target of the tile is written once)
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
-    const int tgt = s_tgt[col0 + 16 * cb];
+    const int tgt = s_tgt[r16 + 16 * cb];
     if (tgt < 0) continue;
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = 32 * wr + 16 * rb + 4 * kq + r;
+        const int row = 32 * wave + 16 * rb + 4 * kq + r;
         Lt[static_cast<int64_t>(tgt) * KP + row] = acc[rb][cb][r];
       }
   }
@@ -979,6 +1115,7 @@ struct FmmTables {
   int P = 0, KP = 0;
   int nslotF = 0, nslotH = 0;
   DeviceBuffer T;         // [noff][KP][KP] column-major
+  DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP rows][KP] row-major
   DeviceBuffer W_m2m;     // [8][KP][KP]
   DeviceBuffer W_l2l;     // [8][KP][KP]
   DeviceBuffer slot_tab;  // [2][8][nslot_max] table index per slot (-1 unused)
@@ -1022,6 +1159,12 @@ struct FmmState {
 void destroy_fmm(FmmState *s) { delete s; }
 
 namespace {
+
+uint16_t bf16_bits_rne_host(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, sizeof(u));
+  return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
 
 int kp_for(int P) {
   const int nd = (P + 1) * (P + 1);
@@ -1075,11 +1218,24 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
       hoh[b * t->nslotH + k] = oh[b][k];
     }
   }
-  // M2L tables
+  // M2L tables: FP32 column-major, and BF16 parts row-major
   std::vector<float> T(static_cast<size_t>(uniq.size()) * KP * KP);
+  std::vector<uint16_t> Tb(static_cast<size_t>(uniq.size()) * 3 * KP * KP);
   for (size_t u = 0; u < uniq.size(); ++u) {
     const std::vector<double> m = m2l_matrix(P, KP, uniq[u].x, uniq[u].y, uniq[u].z);
     for (size_t e = 0; e < m.size(); ++e) T[u * KP * KP + e] = static_cast<float>(m[e]);
+    for (int row = 0; row < KP; ++row)
+      for (int col = 0; col < KP; ++col) {
+        double r = m[static_cast<size_t>(col) * KP + row];
+        for (int p = 0; p < 3; ++p) {
+          const uint16_t bits = bf16_bits_rne_host(static_cast<float>(r));
+          Tb[((u * 3 + p) * KP + row) * KP + col] = bits;
+          uint32_t w = static_cast<uint32_t>(bits) << 16;
+          float f;
+          std::memcpy(&f, &w, sizeof(f));
+          r -= static_cast<double>(f);
+        }
+      }
   }
   std::vector<float> W1(8 * KP * KP), W2(8 * KP * KP);
   for (int oct = 0; oct < 8; ++oct) {
@@ -1102,6 +1258,7 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
     return Status::Ok();
   };
   MF_TRY(upload(t->T, T.data(), T.size() * sizeof(float)));
+  MF_TRY(upload(t->Tb, Tb.data(), Tb.size() * sizeof(uint16_t)));
   MF_TRY(upload(t->W_m2m, W1.data(), W1.size() * sizeof(float)));
   MF_TRY(upload(t->W_l2l, W2.data(), W2.size() * sizeof(float)));
   MF_TRY(upload(t->slot_tab, slots.data(), slots.size() * sizeof(int32_t)));
@@ -1407,6 +1564,38 @@ void fmm_info(const mf_plan *plan, mf_plan_info *info) {
 
 namespace {
 
+template <int KP>
+Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
+                  int ntiles, const int32_t *src, int nslot) {
+  if (ntiles <= 0) return Status::Ok();
+  const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
+#define MF_M2L_ARGS                                                                           \
+  ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tables, tiles, ntiles,                     \
+      F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld
+  switch (prec) {
+    case MF_M2L_FP32: {
+      const void *tables = tab.T.get();
+      hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_M2L_ARGS);
+      break;
+    }
+    case MF_M2L_BF16X6: {
+      const void *tables = tab.Tb.get();
+      hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_M2L_ARGS);
+      break;
+    }
+    case MF_M2L_BF16X3: {
+      const void *tables = tab.Tb.get();
+      hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
+      break;
+    }
+    default:
+      return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
+  }
+#undef MF_M2L_ARGS
+  MF_LAUNCH_CHECK();
+  return Status::Ok();
+}
+
 template <int P, int KP>
 Status upward_and_m2l(mf_plan *plan, bool hess) {
   mf_ctx *ctx = plan->ctx;
@@ -1436,39 +1625,19 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
     }
   }
   {
-    KernelTimer timer(ctx, "m2l");
-    const int ntiles = hess ? F.ntilesH : F.ntilesF;
-    if (ntiles > 0) {
-      // F lists for levels < D; at the leaf level the force uses the H lists,
-      // whose source slots live in srcH (indexed by leaf = global index).
-      if (!hess) {
-        hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((ntiles + 7) / 8)), dim3(2 * KP), 0,
-                           ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                           F.tilesF.as<M2LTile>(), ntiles, F.class_ids2.as<int32_t>(),
-                           F.srcF.as<int32_t>(), tab.nslotF, tab.slot_tab.as<int32_t>(), tab.slot_ld);
-        MF_LAUNCH_CHECK();
-      } else {
-        // split: leaf-level tiles read srcH; others read srcF.  Tiles are ordered
-        // by level, so the leaf tiles are the trailing ones.
-        const int first_leaf_tile = F.leaf_tile_start;
-        if (first_leaf_tile > 0) {
-          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((first_leaf_tile + 7) / 8)), dim3(2 * KP),
-                             0, ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                             F.tilesH.as<M2LTile>(), first_leaf_tile, F.class_ids2.as<int32_t>(),
-                             F.srcF.as<int32_t>(), tab.nslotF, tab.slot_tab.as<int32_t>(),
-                             tab.slot_ld);
-          MF_LAUNCH_CHECK();
-        }
-        if (ntiles - first_leaf_tile > 0) {
-          const int nl = ntiles - first_leaf_tile;
-          hipLaunchKernelGGL((m2l_kernel<KP>), dim3(8 * ((nl + 7) / 8)), dim3(2 * KP), 0,
-                             ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tab.T.as<float>(),
-                             F.tilesH.as<M2LTile>() + first_leaf_tile, nl, F.class_ids2.as<int32_t>(),
-                             F.srcH.as<int32_t>(), tab.nslotH, tab.slot_tab.as<int32_t>(),
-                             tab.slot_ld);
-          MF_LAUNCH_CHECK();
-        }
-      }
+    KernelTimer timer(ctx, hess ? "m2l_force" : "m2l");
+    const int prec = plan->m2l_precision();
+    if (!hess) {
+      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
+                            tab.nslotF));
+    } else {
+      // Tiles are in level order: the leaf level's (trailing) tiles read the
+      // force lists srcH (indexed by leaf = global index), the others srcF.
+      const int lt = F.leaf_tile_start;
+      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
+                            tab.nslotF));
+      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
+                            F.srcH.as<int32_t>(), tab.nslotH));
     }
   }
   {

@@ -29,6 +29,10 @@ struct mf_plan {
   mf::DeviceBuffer pos64;
   bool pos64_valid = false;
   std::unique_ptr<mf::FmmState, mf::FmmDeleter> fmm;
+  // M2L precision of the FMM products; >= 0 overrides opts.m2l_precision
+  // (the mixed-precision solve switches its inner products this way).
+  int m2l_override = -1;
+  int m2l_precision() const { return m2l_override >= 0 ? m2l_override : opts.m2l_precision; }
 };
 
 namespace mf {

@@ -38,6 +38,9 @@ MF_ERR_NO_DEVICE = 6
 
 MODE_ALLPAIRS = 0
 MODE_FMM = 1
+M2L_FP32 = 0
+M2L_BF16X6 = 1
+M2L_BF16X3 = 2
 SOLVE_CG = 0
 SOLVE_GMRES = 1
 SOLVE_REF_NLCG = 2

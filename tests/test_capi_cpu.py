@@ -44,6 +44,23 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(magforce.SolveStats) == 32
 
 
+def test_enum_constants_match_header():
+    """Every MF_ enumerator of the header has the same value in the mirror."""
+    import re
+    text = open(magforce.HEADER_PATH).read()
+    found = dict((k, int(v)) for k, v in re.findall(r"\b(MF_[A-Z0-9_]+)\s*=\s*(\d+)", text))
+    mirror = {"MF_MODE_": "MODE_", "MF_M2L_": "M2L_", "MF_SOLVE_": "SOLVE_", "MF_ERR_": "MF_ERR_",
+              "MF_OK": "MF_OK"}
+    checked = 0
+    for name, value in found.items():
+        for cpref, ppref in mirror.items():
+            if name.startswith(cpref):
+                py = ppref + name[len(cpref):]
+                assert getattr(magforce, py) == value, name
+                checked += 1
+    assert checked >= 13
+
+
 def test_splitmix64_known_answers():
     # SplitMix64 with state 0: published first outputs of the reference generator.
     z = configs.splitmix64(0, 3)

@@ -57,6 +57,31 @@ def test_fmm_force_matches_oracle(ctx, slab_case, order):
     assert rel_err(got, force) < TOL[order]
 
 
+# BF16-split M2L: six products match FP32 to ~1e-7 of the far field; three
+# products leave ~2^-16 per translation, a few 1e-6 of the total field.
+BF16_TOL = {"bf16x6": TOL, "bf16x3": {6: 5e-4, 8: 4e-5, 10: 2e-5}}
+
+
+@pytest.mark.parametrize("order", [6, 8, 10])
+@pytest.mark.parametrize("prec", ["bf16x6", "bf16x3"])
+def test_fmm_bf16_m2l_matches_oracle(ctx, slab_case, order, prec):
+    import magforce
+    pos, m, field, force = slab_case
+    code = {"bf16x6": magforce.M2L_BF16X6, "bf16x3": magforce.M2L_BF16X3}[prec]
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order,
+                         m2l_precision=code)
+    ref32 = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order)
+    mm = dev(m)
+    g, g32 = host(plan.field(mm)), host(ref32.field(mm))
+    F, F32 = host(plan.force(mm)), host(ref32.force(mm))
+    assert rel_err(g, field) < BF16_TOL[prec][order]
+    assert rel_err(F, force) < BF16_TOL[prec][order]
+    if prec == "bf16x6":
+        # same truncation as the FP32 plan, rounding differences only
+        assert rel_err(g, g32) < 2e-7
+        assert rel_err(F, F32) < 2e-7
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (2, 2, 2), (3, 7, 2), (9, 9, 9)])
 def test_fmm_small_and_shallow_trees(ctx, oracle, shape):
     """Trees of one or two levels have no far field (exact near field only);
