@@ -225,8 +225,9 @@ def cpu_baseline(system, budget_s, matvecs_per_step):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the sharded "
-              f"design is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the path is "
+              f"measured on one GPU's share and multi-GPU sharding is not implemented",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import magforce

@@ -5,7 +5,7 @@
 // misspelt evaluate_fitted_near_foece) and public data members.  The bodies
 // (magnetic-force_amd/csrc/magnetic_force.cpp) call the C ABI in mf_capi.h.
 //
-// Differences a caller can observe (each also listed in INTEGRATION.md):
+// Differences a caller can observe:
 //  * the fmmtl-typed members keep their names; the position/charge vectors are
 //    std::vector<float3> and the evaluators are opaque handles (mf_evaluator);
 //  * total_B_over_mu0 starts at zero (the reference left it uninitialised);
