@@ -54,10 +54,13 @@ def parse():
     p.add_argument("--rtol", type=float, default=1e-6)
     p.add_argument("--m2l", default="bf16x3", choices=["fp32", "bf16x6", "bf16x3"],
                    help="FMM M2L precision (bf16x3: the mixed-precision FMM)")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the 96-droplet batch leg")
     p.add_argument("--no-compare", action="store_true",
                    help="skip the FP32-M2L FMM run of the same workload")
     p.add_argument("--no-check", action="store_true")
-    p.add_argument("--check-samples", type=int, default=4096)
+    p.add_argument("--check-samples", type=int, default=0,
+                   help="subsample size of the error check (default 4096; 65536 for n > 2M)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-allpairs", action="store_true")
@@ -184,6 +187,117 @@ def check_error(ctx, st: Stepper, samples: int):
            "samples": k, "ref_rel_residual": ref.stats["rel_residual_true"],
            "ref_seconds": time.perf_counter() - t0}
     log(f"check: {out}")
+    return out
+
+
+def check_error_subsample(ctx, st: Stepper, samples: int):
+    """Error check for n too large for an FP64 all-pairs solve, on a seeded
+    subsample S of the particles:
+      force_op_rel_l2  ||F - F64|| / ||F64|| over S, F64 the FP64 all-pairs
+                       force (all n sources) at the GPU's own moments: the
+                       error of the force product;
+      residual_rel_l2  ||r|| / ||b|| over S with r = b - x + kappa G x in FP64
+                       (all n sources): the solve's true residual.  With
+                       cond(I - kappa G) <= 1 + chi it bounds the solution
+                       error, and the force error at F* follows from both."""
+    import torch
+    import magforce
+    s = st.s
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(12345)
+    k = min(samples, s.n)
+    idx = torch.tensor(np.sort(rng.choice(s.n, size=k, replace=False)), dtype=torch.int64,
+                       device="cuda")
+    pos64 = st.pos.double()
+    m64 = s.kappa * st.x.double()
+    F_ref = magforce.force_subset_f64(ctx, pos64, m64, s.h, idx)
+    F = st.F[:, idx].double()
+    op_err = float(torch.linalg.norm(F - F_ref) / torch.linalg.norm(F_ref))
+    rms = torch.sqrt((F_ref ** 2).sum(0).mean())
+    op_max = float((F - F_ref).norm(dim=0).max() / rms)
+    g = magforce.field_allpairs_f64(ctx, pos64[:, idx].contiguous(), pos64, m64, s.h)
+    bs = st.b[:, idx].double()
+    r = bs - st.x[:, idx].double() + g
+    res = float(torch.linalg.norm(r) / torch.linalg.norm(bs))
+    out = {"method": "subsample", "force_op_rel_l2": op_err, "force_op_max_over_rms": op_max,
+           "residual_rel_l2": res, "x_err_bound": (1.0 + s.chi) * res, "samples": k,
+           "ref_seconds": time.perf_counter() - t0}
+    log(f"check: {out}")
+    return out
+
+
+def run_droplets(ctx, args, m2l_code):
+    """The 96 uneven droplets (2,048..262,144 particles each, chi in [0.5, 3],
+    random unit H directions) as one batch plan: one step = batch plan
+    (re)build + batched CG to rtol per system + block-diagonal force.  Error:
+    per droplet, up to 128 seeded particles, FP64 force at the GPU moments
+    (that droplet's sources) and the FP64 residual b - x + kappa G x."""
+    import torch
+    import magforce
+    from magforce import configs
+    systems = configs.droplets96()
+    off = [0]
+    for sy in systems:
+        off.append(off[-1] + sy.n)
+    pos = to_dev(np.concatenate([sy.pos for sy in systems]))
+    b = to_dev(np.concatenate([np.tile(np.asarray(sy.hconst, np.float64), (sy.n, 1))
+                               for sy in systems]))
+    kappas = [sy.kappa for sy in systems]
+    kpart = torch.tensor(np.concatenate([np.full(sy.n, sy.kappa, np.float32) for sy in systems]),
+                         device="cuda")
+    state = {"plan": None, "x": None, "F": torch.empty_like(b), "stats": None}
+
+    def step():
+        if state["plan"] is None:
+            state["plan"] = magforce.BatchPlan(ctx, pos, off, systems[0].h, m2l_precision=m2l_code)
+        else:
+            state["plan"].update_positions(pos)
+        res = state["plan"].solve(kappas, b, rtol=args.rtol)
+        state["x"], state["stats"] = res.x, res.stats
+        state["plan"].force(res.x * kpart, out=state["F"])
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    st = state["stats"]
+    out = {"workload": "droplets96", "systems": len(systems), "n": off[-1],
+           "n_min": min(sy.n for sy in systems), "n_max": max(sy.n for sy in systems),
+           "ms_per_step": ms, "m2l_precision": args.m2l,
+           "solver": {"iterations": st["iterations"], "matvecs": st["matvecs"],
+                      "converged": st["converged"],
+                      "max_sys_rel_residual": max(st["sys_rel_residual"])},
+           "plan": state["plan"].info()}
+    if not args.no_check:
+        t0 = time.perf_counter()
+        rng = np.random.default_rng(4321)
+        dF = nF = dr = nb = 0.0
+        pos64 = pos.double()
+        m64 = state["x"].double() * kpart.double()
+        for sy, a, e in zip(systems, off[:-1], off[1:]):
+            k = min(128, sy.n)
+            loc = torch.tensor(np.sort(rng.choice(sy.n, size=k, replace=False)), device="cuda")
+            p64, mm = pos64[:, a:e].contiguous(), m64[:, a:e].contiguous()
+            F_ref = magforce.force_subset_f64(ctx, p64, mm, sy.h, loc)
+            F = state["F"][:, a:e][:, loc].double()
+            dF += float(((F - F_ref) ** 2).sum())
+            nF += float((F_ref ** 2).sum())
+            g = magforce.field_allpairs_f64(ctx, p64[:, loc].contiguous(), p64, mm, sy.h)
+            bs = b[:, a:e][:, loc].double()
+            r = bs - state["x"][:, a:e][:, loc].double() + g
+            dr += float((r ** 2).sum())
+            nb += float((bs ** 2).sum())
+        out["check"] = {"method": "subsample per droplet (<=128 each)",
+                        "force_op_rel_l2": (dF / nF) ** 0.5, "residual_rel_l2": (dr / nb) ** 0.5,
+                        "ref_seconds": time.perf_counter() - t0}
+    state["plan"].close()
+    log(f"droplets96: {out}")
     return out
 
 
@@ -330,6 +444,7 @@ def main():
     if mode == magforce.MODE_FMM and args.m2l != "fp32" and not args.no_compare:
         compare = time_fp32_fmm(ctx, system, args)
         compare["speedup"] = compare["ms_per_step"] / ms_step
+    batch = None if args.no_batch else run_droplets(ctx, args, m2l_code)
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)
     if allpairs is not None:
         # plain FP32 all-pairs step at this n, from the measured pair rates
@@ -337,7 +452,12 @@ def main():
         per_field_ms = float(n) * n / (allpairs["gpairs_per_s"] * 1e9) * 1e3
         allpairs["est_step_ms_at_workload_n"] = per_field_ms * (matvecs + 2)
         allpairs["est_speedup_of_value"] = allpairs["est_step_ms_at_workload_n"] / ms_step
-    check = None if args.no_check else check_error(ctx, st, args.check_samples)
+    check = None
+    if not args.no_check:
+        if n > 2_000_000:
+            check = check_error_subsample(ctx, st, args.check_samples or 65536)
+        else:
+            check = check_error(ctx, st, args.check_samples or 4096)
     cpu = None if args.no_cpu else cpu_baseline(system, args.cpu_seconds, matvecs)
 
     line = {
@@ -353,6 +473,7 @@ def main():
                    "solver": st.stats},
         "error": check,
         "fp32_fmm": compare,
+        "batch": batch,
         "allpairs": allpairs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -233,6 +233,30 @@ int mf_plan_solve(mf_plan *plan, double kappa, const float *bx, const float *by,
 int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3n,
                       const mf_solve_opts *opts, mf_solve_stats *stats);
 
+/* ---- batch plans: many independent particle systems in one plan ---------- */
+/* nsys systems stored back to back: system s owns particles
+ * [offsets[s], offsets[s + 1]) of x, y, z (offsets: host, nsys + 1 entries,
+ * offsets[0] = 0).  Systems do not interact: every product of the plan is
+ * block-diagonal.  FMM mode only (opts->mode = MF_MODE_FMM); one smoothing
+ * length h for all systems.  The systems share one octree forest (the system
+ * index sits above the Morton bits of every key), so each kernel of a product
+ * is one launch for the whole batch.  mf_plan_update_positions keeps nsys and
+ * the offsets (same total n).  No counterpart in the reference, which solves
+ * one particle set per cuda_magnet_cgOptimizer. */
+int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const float *x,
+                         const float *y, const float *z, float h, const mf_plan_opts *opts,
+                         mf_plan **out);
+/* (I - kappa[s] G_s) x = b for every system s at once: conjugate gradients
+ * with per-system scalars in lockstep and one shared product per iteration;
+ * a system stops updating once ||r_s|| <= rtol ||b_s||.  kappa: host, nsys.
+ * opts: method must be MF_SOLVE_CG (rtol, maxit, zero_start honoured).
+ * stats->rel_residual is the largest per-system relative residual;
+ * sys_rel_residual (host, nsys, may be NULL) receives each system's. */
+int mf_plan_solve_batch(mf_plan *plan, const double *kappa, const float *bx, const float *by,
+                        const float *bz, float *xx, float *xy, float *xz,
+                        const mf_solve_opts *opts, mf_solve_stats *stats,
+                        double *sys_rel_residual);
+
 /* ---- fused SoA 3-vector utilities (cuda_abstract_vector.cuh:12-161, 502-563) */
 /* out = alpha a + beta b (any of a, b may alias out; b may be NULL => beta ignored) */
 int mf_vec_axpby(mf_ctx *ctx, int64_t n, float alpha, const float *ax, const float *ay,

@@ -1,7 +1,9 @@
 // mf_capi.cpp -- extern "C" entry points (include/mf_capi.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -100,6 +102,8 @@ Status plan_set_positions(mf_plan *plan, int64_t n, const float *x, const float 
   mf_ctx *ctx = plan->ctx;
   MF_ARG(n >= 0, "n must be >= 0");
   MF_ARG(n == 0 || (x && y && z), "positions must be non-null");
+  MF_ARG(plan->nsys == 1 || n == plan->sys_offsets.back(),
+         "a batch plan keeps its particle count (offsets[nsys])");
   plan->n = n;
   plan->pos64_valid = false;
   const int64_t np = padded_sources(n);
@@ -460,6 +464,68 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
   return mf::finish(ctx, run());
 }
 
+int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const float *x,
+                         const float *y, const float *z, float h, const mf_plan_opts *opts,
+                         mf_plan **out) {
+  if (!ctx || !out) return mf::finish(ctx, Status::Err(MF_ERR_INVALID_ARG, "ctx/out is null"));
+  *out = nullptr;
+  auto run = [&]() -> Status {
+    MF_ARG(nsys >= 1 && offsets, "nsys must be >= 1 with nsys + 1 offsets");
+    MF_ARG(offsets[0] == 0, "offsets[0] must be 0");
+    for (int s = 0; s < nsys; ++s) MF_ARG(offsets[s + 1] >= offsets[s], "offsets must not decrease");
+    MF_ARG(offsets[nsys] <= INT32_MAX, "batch too large");
+    MF_ARG(opts && opts->mode == MF_MODE_FMM, "batch plans need mode MF_MODE_FMM");
+    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X3,
+           "unknown M2L precision");
+    MF_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<mf_plan> plan(new (std::nothrow) mf_plan());
+    if (!plan) return Status::Err(MF_ERR_ALLOC, "out of host memory");
+    plan->ctx = ctx;
+    plan->opts = *opts;
+    plan->h = h;
+    plan->fc = mf::make_field_consts(h);
+    plan->hc = mf::make_hess_consts(h);
+    plan->nsys = nsys;
+    plan->sys_offsets.assign(offsets, offsets + nsys + 1);
+    const int64_t n = offsets[nsys];
+    std::vector<int32_t> sys(static_cast<size_t>(n));
+    for (int s = 0; s < nsys; ++s)
+      for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i) sys[static_cast<size_t>(i)] = s;
+    MF_TRY(plan->sys_of.reserve(sizeof(int32_t) * static_cast<size_t>(n > 0 ? n : 1)));
+    if (n > 0)
+      MF_HIP(hipMemcpy(plan->sys_of.get(), sys.data(), sizeof(int32_t) * sys.size(),
+                       hipMemcpyHostToDevice));
+    MF_TRY(mf::plan_set_positions(plan.get(), n, x, y, z));
+    *out = plan.release();
+    return Status::Ok();
+  };
+  return mf::finish(ctx, run());
+}
+
+int mf_plan_solve_batch(mf_plan *plan, const double *kappa, const float *bx, const float *by,
+                        const float *bz, float *xx, float *xy, float *xz,
+                        const mf_solve_opts *opts, mf_solve_stats *stats,
+                        double *sys_rel_residual) {
+  if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
+  auto run = [&]() -> Status {
+    MF_ARG(kappa, "kappa (one per system) is null");
+    MF_ARG(plan->n == 0 || (bx && by && bz && xx && xy && xz), "pointers are null");
+    MF_ARG(plan->opts.mode == MF_MODE_FMM, "batch solves need an FMM plan");
+    MF_HIP(hipSetDevice(plan->ctx->device));
+    mf_solve_opts o{};
+    o.method = MF_SOLVE_CG;
+    o.maxit = 1000;
+    o.rtol = 1e-6;
+    if (opts) o = *opts;
+    mf_solve_stats st{};
+    MF_TRY(mf::solve_batch(plan, kappa, bx, by, bz, xx, xy, xz, o, &st, sys_rel_residual));
+    if (stats) *stats = st;
+    return Status::Ok();
+  };
+  return mf::finish(plan->ctx, run());
+}
+
 int mf_plan_update_positions(mf_plan *plan, int64_t n, const float *x, const float *y,
                              const float *z) {
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
@@ -486,11 +552,16 @@ int mf_plan_get_info(const mf_plan *plan, mf_plan_info *info) {
   info->n = plan->n;
   info->mode = plan->opts.mode;
   if (plan->opts.mode == MF_MODE_FMM && plan->fmm) {
-    mf::fmm_info(plan, info);
-  } else {
-    info->p2p_field_pairs = plan->n * plan->n;
-    info->p2p_force_pairs = plan->n * plan->n;
+    // the plan is logically const: fmm_info only fills its cached work counts
+    mf_plan *p = const_cast<mf_plan *>(plan);
+    auto run = [&]() -> Status {
+      MF_HIP(hipSetDevice(p->ctx->device));
+      return mf::fmm_info(p, info);
+    };
+    return mf::finish(p->ctx, run());
   }
+  info->p2p_field_pairs = plan->n * plan->n;
+  info->p2p_force_pairs = plan->n * plan->n;
   return MF_OK;
 }
 

@@ -106,14 +106,34 @@ __device__ inline int find_key(const uint64_t *__restrict__ keys, int count, uin
   return -1;
 }
 
+// Box geometry of one level.  Keys carry the system index above the 3l
+// Morton bits (a plan may hold several independent particle systems, each
+// with its own root cube of the common side L); lo[3 sys + a] is that
+// system's root corner.
+struct Frame {
+  const double *__restrict__ lo;
+  int shift;  // 3 l
+};
+
+__device__ inline void box_centre(const Frame f, uint64_t key, double s, double &cx, double &cy,
+                                  double &cz) {
+  const uint64_t sys = key >> f.shift;
+  const I3 c = demorton(key & ((1ULL << f.shift) - 1ULL));
+  cx = f.lo[3 * sys] + (c.x + 0.5) * s;
+  cy = f.lo[3 * sys + 1] + (c.y + 0.5) * s;
+  cz = f.lo[3 * sys + 2] + (c.z + 0.5) * s;
+}
+
 // ---------------------------------------------------------------------------
 // Tree build kernels.
 
-__global__ void bbox_partial_kernel(int64_t n, const SrcRec *__restrict__ rec, float *__restrict__ part) {
+// Bounding box of each chunk [chunk[2c], chunk[2c + 1]) of the records.
+__global__ void bbox_partial_kernel(const int64_t *__restrict__ chunk, const SrcRec *__restrict__ rec,
+                                    float *__restrict__ part) {
   __shared__ float sh[6][256];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x; i < n;
-       i += static_cast<int64_t>(gridDim.x) * 256) {
+  const int64_t c0 = chunk[2 * blockIdx.x], c1 = chunk[2 * blockIdx.x + 1];
+  for (int64_t i = c0 + threadIdx.x; i < c1; i += 256) {
     const SrcRec r = rec[i];
     lo[0] = fminf(lo[0], r.x);
     lo[1] = fminf(lo[1], r.y);
@@ -138,19 +158,23 @@ __global__ void bbox_partial_kernel(int64_t n, const SrcRec *__restrict__ rec, f
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void keys_kernel(int64_t n, const SrcRec *__restrict__ rec, double lox, double loy,
-                            double loz, double inv_s, int cells, uint64_t *__restrict__ keys,
+// key = system << 3D | Morton(leaf cell); sys_of == nullptr: one system.
+__global__ void keys_kernel(int64_t n, const SrcRec *__restrict__ rec,
+                            const int32_t *__restrict__ sys_of, const double *__restrict__ sys_lo,
+                            double inv_s, int cells, int shift, uint64_t *__restrict__ keys,
                             int32_t *__restrict__ idx) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SrcRec r = rec[i];
+  const int sys = sys_of ? sys_of[i] : 0;
   int c[3];
-  const double p[3] = {(r.x - lox) * inv_s, (r.y - loy) * inv_s, (r.z - loz) * inv_s};
+  const double p[3] = {(r.x - sys_lo[3 * sys]) * inv_s, (r.y - sys_lo[3 * sys + 1]) * inv_s,
+                       (r.z - sys_lo[3 * sys + 2]) * inv_s};
   for (int a = 0; a < 3; ++a) {
     int v = static_cast<int>(floor(p[a]));
     c[a] = v < 0 ? 0 : (v >= cells ? cells - 1 : v);
   }
-  keys[i] = morton(c[0], c[1], c[2]);
+  keys[i] = (static_cast<uint64_t>(sys) << shift) | morton(c[0], c[1], c[2]);
   idx[i] = static_cast<int32_t>(i);
 }
 
@@ -205,36 +229,40 @@ __global__ void set_parent_kernel(int nchild, const int32_t *__restrict__ child_
 
 // Near-field lists of the leaves: list[b * nslot + k] = leaf index or -1.
 __global__ void near_list_kernel(int nleaf, const uint64_t *__restrict__ leaf_keys, int cells,
-                                 const int3 *__restrict__ offs, int nslot,
+                                 int shift, const int3 *__restrict__ offs, int nslot,
                                  int32_t *__restrict__ list) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<int64_t>(nleaf) * nslot) return;
   const int b = static_cast<int>(t / nslot), k = static_cast<int>(t % nslot);
-  const I3 c = demorton(leaf_keys[b]);
+  const uint64_t key = leaf_keys[b];
+  const uint64_t sys_bits = (key >> shift) << shift;
+  const I3 c = demorton(key & ((1ULL << shift) - 1ULL));
   const int3 o = offs[k];
   const int x = c.x + o.x, y = c.y + o.y, z = c.z + o.z;
   int r = -1;
   if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells)
-    r = find_key(leaf_keys, nleaf, morton(x, y, z));
+    r = find_key(leaf_keys, nleaf, sys_bits | morton(x, y, z));
   list[t] = r;
 }
 
 // M2L source slots of the boxes of one level: src[b * nslot + k] = global
 // index of the source box at offset offs[parity][k] (target - source), or -1.
 __global__ void m2l_list_kernel(int nbox, int box_base, const uint64_t *__restrict__ keys,
-                                int cells, const int3 *__restrict__ offs, int nslot,
+                                int cells, int shift, const int3 *__restrict__ offs, int nslot,
                                 int32_t *__restrict__ src) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<int64_t>(nbox) * nslot) return;
   const int b = static_cast<int>(t / nslot), k = static_cast<int>(t % nslot);
-  const I3 c = demorton(keys[b]);
+  const uint64_t key = keys[b];
+  const uint64_t sys_bits = (key >> shift) << shift;
+  const I3 c = demorton(key & ((1ULL << shift) - 1ULL));
   const int parity = (c.x & 1) | ((c.y & 1) << 1) | ((c.z & 1) << 2);
   const int3 o = offs[parity * nslot + k];
   int r = -1;
   if (o.x != 0x7fffffff) {
     const int x = c.x - o.x, y = c.y - o.y, z = c.z - o.z;
     if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells) {
-      const int f = find_key(keys, nbox, morton(x, y, z));
+      const int f = find_key(keys, nbox, sys_bits | morton(x, y, z));
       r = f >= 0 ? box_base + f : -1;
     }
   }
@@ -262,15 +290,14 @@ template <int P, int KP>
 __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__restrict__ start,
                                                  const int32_t *__restrict__ count,
                                                  const uint64_t *__restrict__ keys,
-                                                 const SrcRec *__restrict__ rec, double lox,
-                                                 double loy, double loz, double s,
-                                                 float *__restrict__ Mt) {
+                                                 const SrcRec *__restrict__ rec, Frame fr,
+                                                 double s, float *__restrict__ Mt) {
   constexpr int ND = (P + 1) * (P + 1);
   __shared__ float red[64][ND + 1];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
-  const I3 c = demorton(keys[b]);
-  const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
+  double cx, cy, cz;
+  box_centre(fr, keys[b], s, cx, cy, cz);
   const float inv_s = static_cast<float>(1.0 / s);
   float acc[ND];
 #pragma unroll
@@ -432,10 +459,6 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t (&p)[NPART]) {
 
 constexpr int kGatherIt = kTileT / 8;  // 64 * KP / 4 / (2 KP)
 
-__device__ __forceinline__ int next_slot(const int32_t *__restrict__ smap, int s, int nslot) {
-  while (s < nslot && smap[s] < 0) ++s;
-  return s;
-}
 
 template <int KP>
 __device__ __forceinline__ void m2l_src_load(const int *s_tgt, const int32_t *__restrict__ src_slots,
@@ -475,7 +498,10 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
                                                           const int32_t *__restrict__ src_slots,
                                                           int src_ld,
                                                           const int32_t *__restrict__ slot_table,
-                                                          int slot_ld) {
+                                                          int slot_ld,
+                                                          const int32_t *__restrict__ tile_list,
+                                                          const int32_t *__restrict__ tile_nlist,
+                                                          int list_ld) {
   constexpr int NT = 2 * KP;
   constexpr int LDF = KP + 4;  // FP32 row pitch (floats)
   constexpr int LDB = KP + 8;  // BF16 row pitch (elements)
@@ -499,18 +525,18 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
   const int32_t *__restrict__ smap = slot_table + static_cast<int64_t>(tile.slots) * slot_ld;
   const int r16 = lane & 15;
   const int kq = lane >> 4;
-  int cur = next_slot(smap, 0, tile.nslot);
+  // the tile's non-empty offsets (plan build), walked in slot order
+  const int32_t *__restrict__ tl = tile_list + static_cast<int64_t>(tile_id) * list_ld;
+  const int na = tile_nlist[tile_id];
   int src[kGatherIt];
   float4 v[kGatherIt];
   int any = 0;
-  int nxt = tile.nslot;
-  if (cur < tile.nslot) {
-    m2l_src_load<KP>(s_tgt, src_slots, src_ld, cur, tid, src);
+  if (na > 0) {
+    m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[0], tid, src);
     any = m2l_mult_load<KP>(Mt, src, tid, v);
-    nxt = next_slot(smap, cur + 1, tile.nslot);
-    if (nxt < tile.nslot) m2l_src_load<KP>(s_tgt, src_slots, src_ld, nxt, tid, src);
+    if (na > 1) m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[1], tid, src);
   }
-  while (cur < tile.nslot) {
+  for (int pc = 0; pc < na; ++pc) {
     __syncthreads();  // the previous offset's operands are consumed
 #pragma unroll
     for (int it = 0; it < kGatherIt; ++it) {
@@ -533,11 +559,10 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
       }
     }
     const int go = __syncthreads_or(any);
-    const int tix = smap[cur];
-    const int nn = nxt < tile.nslot ? next_slot(smap, nxt + 1, tile.nslot) : tile.nslot;
-    if (nxt < tile.nslot) {
+    const int tix = smap[tl[pc]];
+    if (pc + 1 < na) {
       any = m2l_mult_load<KP>(Mt, src, tid, v);
-      if (nn < tile.nslot) m2l_src_load<KP>(s_tgt, src_slots, src_ld, nn, tid, src);
+      if (pc + 2 < na) m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[pc + 2], tid, src);
     }
     if (go) {
       if constexpr (NPART == 0) {
@@ -591,8 +616,6 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
         }
       }
     }
-    cur = nxt;
-    nxt = nn;
   }
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb) {
@@ -605,6 +628,36 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
         const int row = 32 * wave + 16 * rb + 4 * kq + r;
         Lt[static_cast<int64_t>(tgt) * KP + row] = acc[rb][cb][r];
       }
+  }
+}
+
+// Non-empty offsets of every M2L tile: list[tile][0 .. nlist[tile]) = the
+// slots (in order) where at least one target of the tile has a source box.
+// One workgroup per tile; nslot <= 1024.
+__global__ __launch_bounds__(256) void tile_slots_kernel(const M2LTile *__restrict__ tiles,
+                                                         const int32_t *__restrict__ sorted_boxes,
+                                                         const int32_t *__restrict__ src_slots,
+                                                         int src_ld,
+                                                         const int32_t *__restrict__ slot_table,
+                                                         int slot_ld, int list_ld,
+                                                         int32_t *__restrict__ list,
+                                                         int32_t *__restrict__ nlist) {
+  __shared__ int s_flag[1024];
+  const M2LTile tile = tiles[blockIdx.x];
+  const int32_t *__restrict__ smap = slot_table + static_cast<int64_t>(tile.slots) * slot_ld;
+  for (int k = threadIdx.x; k < tile.nslot; k += 256) s_flag[k] = 0;
+  __syncthreads();
+  for (int j = 0; j < tile.count; ++j) {
+    const int tgt = sorted_boxes[tile.first + j];
+    for (int k = threadIdx.x; k < tile.nslot; k += 256)
+      if (src_slots[static_cast<int64_t>(tgt) * src_ld + k] >= 0) s_flag[k] = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int k = 0; k < tile.nslot; ++k)
+      if (s_flag[k] && smap[k] >= 0) list[static_cast<int64_t>(blockIdx.x) * list_ld + c++] = k;
+    nlist[blockIdx.x] = c;
   }
 }
 
@@ -711,16 +764,12 @@ __device__ __forceinline__ void l2p_hess(const float *__restrict__ Lt, float x, 
 // Near ranges of a leaf: the particle ranges of its near boxes, with boxes of
 // consecutive index merged (leaf b + 1 holds the particles right after leaf
 // b's, so a run of consecutive leaf indices is one contiguous range).  One
-// wave per leaf.  Slots [0, nmix) form class 0 and [nmix, nslot) class 1;
-// ranges never cross classes and keep slot order, so the summation order is
-// fixed.  rng[leaf * nslot + r] = {first, end}; nr[leaf] = {class-0 ranges,
-// class-1 ranges} (class 1 follows class 0 in rng).
-__device__ __forceinline__ bool slot_has(const int *__restrict__ s_box, int lo, int hi, int box) {
-  if (box < 0) return false;
-  for (int k = lo; k < hi; ++k)
-    if (s_box[k] == box) return true;
-  return false;
-}
+// wave per leaf: the slot entries, keyed (class << 28 | box) with slots
+// [0, nmix) class 0 and [nmix, nslot) class 1, are bitonic-sorted in LDS;
+// runs of consecutive keys are the ranges, emitted in key order (class 0
+// first), so the summation order is fixed.  rng[leaf * nslot + r] =
+// {first, end}; nr[leaf] = {class-0 ranges, class-1 ranges}.
+constexpr int kSortN = 256;  // >= kNearH
 
 __global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_t *__restrict__ list,
                                                          int nslot, int nmix,
@@ -728,36 +777,55 @@ __global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_
                                                          const int32_t *__restrict__ count,
                                                          int2 *__restrict__ rng,
                                                          int2 *__restrict__ nr) {
-  __shared__ int s_box[kNearH];
+  __shared__ int s_key[kSortN];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   const int32_t *__restrict__ l = list + static_cast<int64_t>(b) * nslot;
-  for (int k = lane; k < nslot; k += 64) s_box[k] = l[k];
+  for (int k = lane; k < kSortN; k += 64) {
+    int key = 0x7fffffff;
+    if (k < nslot) {
+      const int sb = l[k];
+      if (sb >= 0) key = ((k < nmix ? 0 : 1) << 28) | sb;
+    }
+    s_key[k] = key;
+  }
   __syncthreads();
-  int2 *__restrict__ out = rng + static_cast<int64_t>(b) * nslot;
-  int pos = 0, n0 = 0;
-  for (int cls = 0; cls < 2; ++cls) {
-    const int lo = cls ? nmix : 0, hi = cls ? nslot : nmix;
-    for (int k0 = lo; k0 < hi; k0 += 64) {
-      const int k = k0 + lane;
-      bool is_start = false;
-      int j0 = 0, j1 = 0;
-      if (k < hi) {
-        const int sb = s_box[k];
-        if (sb >= 0 && !slot_has(s_box, lo, hi, sb - 1)) {
-          int e = sb;
-          while (slot_has(s_box, lo, hi, e + 1)) ++e;
-          is_start = true;
-          j0 = first[sb];
-          j1 = first[e] + count[e];
+  // bitonic sort, ascending: kSortN / 2 compare-exchanges per step, 2 per lane
+  for (int size = 2; size <= kSortN; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int c = lane; c < kSortN / 2; c += 64) {
+        const int i = 2 * c - (c & (stride - 1));
+        const int j = i + stride;
+        const bool up = (i & size) == 0;
+        const int a = s_key[i], d = s_key[j];
+        if ((a > d) == up) {
+          s_key[i] = d;
+          s_key[j] = a;
         }
       }
-      const uint64_t mask = __ballot(is_start);
-      const int before = __popcll(mask & ((1ULL << lane) - 1ULL));
-      if (is_start) out[pos + before] = make_int2(j0, j1);
-      pos += __popcll(mask);
+      __syncthreads();
     }
-    if (cls == 0) n0 = pos;
+  int2 *__restrict__ out = rng + static_cast<int64_t>(b) * nslot;
+  int pos = 0, n0 = 0;
+  for (int k0 = 0; k0 < kSortN; k0 += 64) {
+    const int k = k0 + lane;
+    const int key = s_key[k];
+    const bool valid = key != 0x7fffffff;
+    const bool is_start = valid && (k == 0 || s_key[k - 1] != key - 1);
+    int j0 = 0, j1 = 0;
+    if (is_start) {
+      int e = k;
+      while (e + 1 < kSortN && s_key[e + 1] == s_key[e] + 1) ++e;
+      const int sb = key & 0x0fffffff, se = s_key[e] & 0x0fffffff;
+      j0 = first[sb];
+      j1 = first[se] + count[se];
+    }
+    const uint64_t mask = __ballot(is_start);
+    const uint64_t cls0 = __ballot(is_start && (key >> 28) == 0);
+    const int before = __popcll(mask & ((1ULL << lane) - 1ULL));
+    if (is_start) out[pos + before] = make_int2(j0, j1);
+    pos += __popcll(mask);
+    n0 += __popcll(cls0);
   }
   if (lane == 0) nr[b] = make_int2(n0, pos - n0);
 }
@@ -767,13 +835,12 @@ template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_field_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const float *__restrict__ Lt, double lox, double loy,
-    double loz, double s, float hc0, float hc1, float hc2, float *__restrict__ ox,
-    float *__restrict__ oy, float *__restrict__ oz) {
+    const int32_t *__restrict__ perm, const float *__restrict__ Lt, Frame fr, double s, float hc0,
+    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
-  const I3 c = demorton(keys[b]);
-  const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
+  double cx, cy, cz;
+  box_centre(fr, keys[b], s, cx, cy, cz);
   const float inv_s = static_cast<float>(1.0 / s);
   const double far_scale = -1.0 / (4.0 * kPi) / (s * s);
   const int s0 = start[b], cnt = count[b];
@@ -793,14 +860,16 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(
   }
 }
 
-// Near-field source loops over one contiguous source range.  Sources are
-// read with uniform (scalar) loads; each lane owns one target.
+// Near-field source loops over one contiguous source range, taking every
+// STRIDE-th source from j0.  With STRIDE = 1 the source index is uniform
+// over the wave and the records come through scalar loads.
+template <int STRIDE>
 __device__ __forceinline__ void field_range_far(const SrcRec *__restrict__ rec, int j0, int j1,
                                                 float tx, float ty, float tz, float &a1x,
                                                 float &a1y, float &a1z, float &a2x, float &a2y,
                                                 float &a2z) {
 #pragma unroll 2
-  for (int j = j0; j < j1; ++j) {
+  for (int j = j0; j < j1; j += STRIDE) {
     const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
     const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
     const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
@@ -819,13 +888,14 @@ __device__ __forceinline__ void field_range_far(const SrcRec *__restrict__ rec, 
   }
 }
 
+template <int STRIDE>
 __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec, int j0, int j1,
                                                   float tx, float ty, float tz,
                                                   const FieldConsts &fc, float &a1x, float &a1y,
                                                   float &a1z, float &a2x, float &a2y,
                                                   float &a2z) {
 #pragma unroll 2
-  for (int j = j0; j < j1; ++j) {
+  for (int j = j0; j < j1; j += STRIDE) {
     const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
     const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
     const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
@@ -846,36 +916,80 @@ __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec
   }
 }
 
+// Targets [tb, te) of a leaf of cnt particles handled by the T-shaped kernel.
+template <int T>
+__device__ __forceinline__ void p2p_target_span(int cnt, int &tb, int &te) {
+  const int full = cnt & ~63, r = cnt & 63;
+  if (T == 64) {
+    tb = 0;
+    te = r > 32 ? cnt : full;
+  } else {
+    tb = full;
+    te = r <= 32 ? cnt : full;
+  }
+}
+
+// Sum over the 64 / T lane groups of a wave (lanes t, t + T, t + 2T, ...).
+template <int T>
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int off = T; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
-// writes H_const + near (trees without a far field).  One wave per leaf.
+// writes H_const + near (trees without a far field).  One wave per leaf, in
+// one of two shapes:
+//  * T = 64: one target per lane, every lane walks all sources (scalar
+//    loads), over the leaf's full chunks of 64 targets -- and over the
+//    remainder too when it holds more than 32;
+//  * T = 32, 16, 8, for a remainder of at most T targets (a sparse leaf, or
+//    the tail of a leaf of 65..79 particles): the wave is split into
+//    G = 64 / T groups of T lanes; every group holds all remainder targets
+//    and takes every G-th source of each range (vector loads), and the
+//    groups' partial sums are added across lanes.  A short remainder then
+//    keeps (most of) the wave busy instead of idling 64 - r lanes.
+// leaves: the leaf indices this launch covers (blockIdx.x -> leaf).
+template <int T>
 __global__ __launch_bounds__(64) void p2p_field_kernel(
-    int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm, const int2 *__restrict__ rng,
-    const int2 *__restrict__ nr, int nslot, int accumulate, FieldConsts fc, float hc0, float hc1,
-    float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
-  const int b = blockIdx.x;
+    const int32_t *__restrict__ leaves, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
+    const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
+    int nslot, int accumulate, FieldConsts fc, float hc0, float hc1, float hc2,
+    float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
+  constexpr int G = 64 / T;
+  const int b = leaves[blockIdx.x];
   const int lane = threadIdx.x;
+  const int tl = T == 64 ? lane : lane % T;
+  const int g = T == 64 ? 0 : lane / T;
   const int s0 = start[b], cnt = count[b];
+  int tb, te;
+  p2p_target_span<T>(cnt, tb, te);
   const int2 nrb = nr[b];
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double cf = 1.0 / (4.0 * kPi);
-  for (int t0 = 0; t0 < cnt; t0 += 64) {
-    const int t = t0 + lane;
-    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+  for (int t0 = tb; t0 < te; t0 += T) {
+    const int t = t0 + tl;
+    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
     for (int r = 0; r < nrb.x + nrb.y; ++r) {
       const int2 q = rg[r];
       float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
       if (r < nrb.x)
-        field_range_mixed(rec, q.x, q.y, tx, ty, tz, fc, a1x, a1y, a1z, a2x, a2y, a2z);
+        field_range_mixed<G>(rec, q.x + g, q.y, tx, ty, tz, fc, a1x, a1y, a1z, a2x, a2y, a2z);
       else
-        field_range_far(rec, q.x, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
+        field_range_far<G>(rec, q.x + g, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
       acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
       acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
       acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
     }
-    if (t < cnt) {
+    if constexpr (T < 64) {
+      acc0 = group_sum<T>(acc0);
+      acc1 = group_sum<T>(acc1);
+      acc2 = group_sum<T>(acc2);
+    }
+    if (g == 0 && t < te) {
       const int o = perm[s0 + t];
       const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
       const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
@@ -884,6 +998,27 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
       oy[o] = static_cast<float>(b1 + cf * acc1);
       oz[o] = static_cast<float>(b2 + cf * acc2);
     }
+  }
+}
+
+// Leaf shapes for the near-field kernels.  List 0 (T = 64) holds every leaf
+// with a full chunk of 64 or a remainder of more than 32; lists 1..3 hold the
+// leaves whose remainder (cnt mod 64) is 17..32, 9..16 or 1..8.  A leaf can
+// be in list 0 and in one of the others: they cover disjoint targets.
+__global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ count,
+                                       int32_t *__restrict__ lists, int32_t *__restrict__ sizes) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nleaf) return;
+  const int c = count[b];
+  const int r = c & 63;
+  if (c >= 64 || r > 32) {
+    const int at = atomicAdd(&sizes[0], 1);
+    lists[at] = b;
+  }
+  if (r > 0 && r <= 32) {
+    const int k = r > 16 ? 1 : (r > 8 ? 2 : 3);
+    const int at = atomicAdd(&sizes[k], 1);
+    lists[static_cast<int64_t>(k) * nleaf + at] = b;
   }
 }
 
@@ -896,13 +1031,13 @@ template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_force_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const float *__restrict__ Lt, int use_far, double lox,
-    double loy, double loz, double s, Text9 text, int has_text, float *__restrict__ fx,
-    float *__restrict__ fy, float *__restrict__ fz) {
+    const int32_t *__restrict__ perm, const float *__restrict__ Lt, int use_far, Frame fr,
+    double s, Text9 text, int has_text, float *__restrict__ fx, float *__restrict__ fy,
+    float *__restrict__ fz) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
-  const I3 c = demorton(keys[b]);
-  const double cx = lox + (c.x + 0.5) * s, cy = loy + (c.y + 0.5) * s, cz = loz + (c.z + 0.5) * s;
+  double cx, cy, cz;
+  box_centre(fr, keys[b], s, cx, cy, cz);
   const float inv_s = static_cast<float>(1.0 / s);
   const double far_scale = -kMu0 / (4.0 * kPi) / (s * s * s);
   const int s0 = start[b], cnt = count[b];
@@ -942,11 +1077,15 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
 }
 
 // P2P, force: F[perm] += (sum over the leaf's near ranges of B[m_j]) . m_i.
+// Leaf shapes (T) as in p2p_field_kernel.
+template <int T>
 __global__ __launch_bounds__(64) void p2p_force_kernel(
-    int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
-    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm, const int2 *__restrict__ rng,
-    const int2 *__restrict__ nr, int nslot, HessConsts hc, float *__restrict__ fx,
-    float *__restrict__ fy, float *__restrict__ fz) {
+    const int32_t *__restrict__ leaves, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
+    const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
+    int nslot, HessConsts hc, float *__restrict__ fx, float *__restrict__ fy,
+    float *__restrict__ fz) {
+  constexpr int G = 64 / T;
   __shared__ float4 s_tab[12];
   if (threadIdx.x < 12) {
     const float *row = &hc.curve[threadIdx.x / 3][(threadIdx.x % 3) * 4];
@@ -954,22 +1093,26 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   }
   __syncthreads();
   const float *tab = reinterpret_cast<const float *>(s_tab);
-  const int b = blockIdx.x;
+  const int b = leaves[blockIdx.x];
   const int lane = threadIdx.x;
+  const int tl = T == 64 ? lane : lane % T;
+  const int g = T == 64 ? 0 : lane / T;
   const int s0 = start[b], cnt = count[b];
+  int tb, te;
+  p2p_target_span<T>(cnt, tb, te);
   const int nrange = nr[b].x;
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double sfac = 3.0 * kMu0 / (4.0 * kPi);
-  for (int t0 = 0; t0 < cnt; t0 += 64) {
-    const int t = t0 + lane;
-    const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+  for (int t0 = tb; t0 < te; t0 += T) {
+    const int t = t0 + tl;
+    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
     double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
     for (int r = 0; r < nrange; ++r) {
       const int2 q = rg[r];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
 #pragma unroll 2
-      for (int j = q.x; j < q.y; ++j) {
+      for (int j = q.x + g; j < q.y; j += G) {
         const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
         const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
         const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
@@ -1000,28 +1143,36 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
       am2 += mz0;
       an += n0;
     }
-    if (t < cnt) {
+    double f0 = ad0 + am0 + static_cast<double>(nx) * an;
+    double f1 = ad1 + am1 + static_cast<double>(ny) * an;
+    double f2 = ad2 + am2 + static_cast<double>(nz) * an;
+    if constexpr (T < 64) {
+      f0 = group_sum<T>(f0);
+      f1 = group_sum<T>(f1);
+      f2 = group_sum<T>(f2);
+    }
+    if (g == 0 && t < te) {
       const int o = perm[s0 + t];
-      fx[o] = static_cast<float>(static_cast<double>(fx[o]) +
-                                 sfac * (ad0 + am0 + static_cast<double>(nx) * an));
-      fy[o] = static_cast<float>(static_cast<double>(fy[o]) +
-                                 sfac * (ad1 + am1 + static_cast<double>(ny) * an));
-      fz[o] = static_cast<float>(static_cast<double>(fz[o]) +
-                                 sfac * (ad2 + am2 + static_cast<double>(nz) * an));
+      fx[o] = static_cast<float>(static_cast<double>(fx[o]) + sfac * f0);
+      fy[o] = static_cast<float>(static_cast<double>(fy[o]) + sfac * f1);
+      fz[o] = static_cast<float>(static_cast<double>(fz[o]) + sfac * f2);
     }
   }
 }
 
-// Moments into the sorted records: rec[i].m = scale * m[perm[i]].
+// Moments into the sorted records: rec[i].m = scale * weight[perm[i]] * m[perm[i]]
+// (weight: optional per-particle factor in caller order, e.g. kappa per system).
 __global__ void pack_sorted_moments_kernel(int64_t n, const int32_t *__restrict__ perm,
                                            const float *__restrict__ mx,
                                            const float *__restrict__ my,
                                            const float *__restrict__ mz, float scale,
+                                           const float *__restrict__ weight,
                                            SrcRec *__restrict__ rec) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int o = perm[i];
-  reinterpret_cast<float4 *>(rec + i)[1] = make_float4(scale * mx[o], scale * my[o], scale * mz[o], 0.f);
+  const float w = weight ? scale * weight[o] : scale;
+  reinterpret_cast<float4 *>(rec + i)[1] = make_float4(w * mx[o], w * my[o], w * mz[o], 0.f);
 }
 
 // Work counts: near pairs of the leaves and valid M2L slots of a box range.
@@ -1129,7 +1280,9 @@ struct FmmTables {
 struct FmmState {
   int P = 10, KP = 128;
   int D = 0;
-  double lo[3] = {0, 0, 0};
+  int nsys = 1;
+  DeviceBuffer sys_lo;  // double [nsys][3]: root corner of each system
+  DeviceBuffer chunks;  // int64 [nchunk][2]: record ranges of the bounding-box pass
   double L = 0.0, s_leaf = 0.0;
   int64_t n = 0;
   std::shared_ptr<FmmTables> tab;
@@ -1144,9 +1297,12 @@ struct FmmState {
   DeviceBuffer nearF, nearH; // leaf lists
   DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
   DeviceBuffer nrF, nrH;     // int2 range counts per leaf (class 0, class 1)
+  DeviceBuffer leaf_lists;   // int32 [4][nleaf]: leaves of > 32, <= 32, <= 16, <= 8 particles
+  int leaf_list_n[4] = {0, 0, 0, 0};
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
+  DeviceBuffer tlistF, tlistH, tnF, tnH;  // non-empty offsets per tile [ntiles][slot_ld], counts
   int ntilesF = 0, ntilesH = 0;
   int leaf_tile_start = 0;  // first tile of the leaf level (tiles are in level order)
   DeviceBuffer Mt, Lt;
@@ -1154,6 +1310,7 @@ struct FmmState {
   int64_t p2p_force_pairs = 0;
   int64_t m2l_pairs_field = 0;
   int64_t m2l_pairs_force = 0;
+  bool counts_valid = false;
 };
 
 void destroy_fmm(FmmState *s) { delete s; }
@@ -1307,33 +1464,70 @@ Status fmm_build(mf_plan *plan) {
     F.count.assign(1, 0);
     return Status::Ok();
   }
-  // 1. bounding box
-  void *pw = nullptr;
-  const int nb = 256;
-  MF_TRY(workspace(ctx, "fmm_bbox", sizeof(float) * 6 * nb, &pw));
-  hipLaunchKernelGGL(bbox_partial_kernel, dim3(nb), dim3(256), 0, ctx->stream, n, rec_in,
-                     static_cast<float *>(pw));
-  MF_LAUNCH_CHECK();
-  std::vector<float> part(6 * nb);
-  MF_HIP(hipMemcpyAsync(part.data(), pw, sizeof(float) * 6 * nb, hipMemcpyDeviceToHost, ctx->stream));
-  MF_HIP(hipStreamSynchronize(ctx->stream));
-  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int b = 0; b < nb; ++b)
-    for (int c = 0; c < 3; ++c) {
-      lo[c] = std::min<double>(lo[c], part[b * 6 + c]);
-      hi[c] = std::max<double>(hi[c], part[b * 6 + 3 + c]);
+  // 1. bounding box of every system, from per-chunk partials (a chunk never
+  //    spans two systems)
+  const int nsys = plan->nsys;
+  F.nsys = nsys;
+  std::vector<int64_t> off = plan->sys_offsets;
+  if (nsys == 1) off = {0, n};
+  std::vector<int64_t> ch;
+  std::vector<int> ch_sys;
+  constexpr int64_t kChunk = 65536;
+  for (int sy = 0; sy < nsys; ++sy)
+    for (int64_t b0 = off[sy]; b0 < off[sy + 1]; b0 += kChunk) {
+      ch.push_back(b0);
+      ch.push_back(std::min(off[sy + 1], b0 + kChunk));
+      ch_sys.push_back(sy);
     }
-  for (int c = 0; c < 3; ++c)
-    if (!std::isfinite(lo[c]) || !std::isfinite(hi[c]))
-      return Status::Err(MF_ERR_INVALID_ARG, "particle positions must be finite");
-  const double ext = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]}) * (1.0 + 1e-6) + 1e-6;
+  const int nch = static_cast<int>(ch_sys.size());
+  MF_TRY(need<int64_t>(F.chunks, 2 * nch));
+  MF_HIP(hipMemcpyAsync(F.chunks.get(), ch.data(), sizeof(int64_t) * ch.size(), hipMemcpyHostToDevice,
+                        ctx->stream));
+  void *pw = nullptr;
+  MF_TRY(workspace(ctx, "fmm_bbox", sizeof(float) * 6 * nch, &pw));
+  hipLaunchKernelGGL(bbox_partial_kernel, dim3(nch), dim3(256), 0, ctx->stream,
+                     F.chunks.as<int64_t>(), rec_in, static_cast<float *>(pw));
+  MF_LAUNCH_CHECK();
+  std::vector<float> part(6 * nch);
+  MF_HIP(hipMemcpyAsync(part.data(), pw, sizeof(float) * 6 * nch, hipMemcpyDeviceToHost, ctx->stream));
+  MF_HIP(hipStreamSynchronize(ctx->stream));
+  std::vector<double> lo(3 * nsys, INFINITY), hi(3 * nsys, -INFINITY);
+  for (int c = 0; c < nch; ++c)
+    for (int a = 0; a < 3; ++a) {
+      const int sy = ch_sys[c];
+      lo[3 * sy + a] = std::min<double>(lo[3 * sy + a], part[c * 6 + a]);
+      hi[3 * sy + a] = std::max<double>(hi[3 * sy + a], part[c * 6 + 3 + a]);
+    }
+  double ext = 0.0;
+  for (int sy = 0; sy < nsys; ++sy) {
+    if (off[sy + 1] == off[sy]) {  // empty system: any finite frame
+      for (int a = 0; a < 3; ++a) lo[3 * sy + a] = hi[3 * sy + a] = 0.0;
+      continue;
+    }
+    for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(lo[3 * sy + a]) || !std::isfinite(hi[3 * sy + a]))
+        return Status::Err(MF_ERR_INVALID_ARG, "particle positions must be finite");
+      ext = std::max(ext, hi[3 * sy + a] - lo[3 * sy + a]);
+    }
+  }
+  ext = ext * (1.0 + 1e-6) + 1e-6;
   int D = 0;
   while (2.0 * h * std::ldexp(1.0, D) < ext) ++D;
-  if (D > kMaxLevels - 1) return Status::Err(MF_ERR_UNSUPPORTED, "particle set too wide for the octree depth");
+  int sys_bits = 0;
+  while ((1LL << sys_bits) < nsys) ++sys_bits;
+  if (D > kMaxLevels - 1 || 3 * D + sys_bits > 63)
+    return Status::Err(MF_ERR_UNSUPPORTED, "particle set too wide for the octree depth");
   F.D = D;
   F.L = 2.0 * h * std::ldexp(1.0, D);
   F.s_leaf = 2.0 * h;
-  for (int c = 0; c < 3; ++c) F.lo[c] = 0.5 * (lo[c] + hi[c]) - 0.5 * F.L;
+  std::vector<double> root(3 * nsys);
+  for (int sy = 0; sy < nsys; ++sy)
+    for (int a = 0; a < 3; ++a)
+      root[3 * sy + a] = 0.5 * (lo[3 * sy + a] + hi[3 * sy + a]) - 0.5 * F.L;
+  MF_TRY(need<double>(F.sys_lo, 3 * nsys));
+  MF_HIP(hipMemcpyAsync(F.sys_lo.get(), root.data(), sizeof(double) * root.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  MF_HIP(hipStreamSynchronize(ctx->stream));  // host vectors go out of scope
   const int cells = 1 << D;
   // 2. keys and sort
   MF_TRY(need<uint64_t>(F.keys, n));
@@ -1344,11 +1538,14 @@ Status fmm_build(mf_plan *plan) {
   MF_TRY(need<int32_t>(F.scan, n + 1));
   MF_TRY(need<SrcRec>(F.rec, n));
   hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, rec_in,
-                     F.lo[0], F.lo[1], F.lo[2], 1.0 / F.s_leaf, cells, F.keys.as<uint64_t>(),
+                     nsys > 1 ? plan->sys_of.as<int32_t>() : static_cast<const int32_t *>(nullptr),
+                     F.sys_lo.as<double>(), 1.0 / F.s_leaf, cells, 3 * D, F.keys.as<uint64_t>(),
                      F.idx.as<int32_t>());
   MF_LAUNCH_CHECK();
+  const int key_bits = 3 * D + sys_bits;
   MF_TRY(sort_pairs_u64_i32(ctx, F.keys.as<uint64_t>(), F.keys_sorted.as<uint64_t>(),
-                            F.idx.as<int32_t>(), F.perm.as<int32_t>(), n, 0, 3 * D > 0 ? 3 * D : 1));
+                            F.idx.as<int32_t>(), F.perm.as<int32_t>(), n, 0,
+                            key_bits > 0 ? key_bits : 1));
   hipLaunchKernelGGL(gather_sorted_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n,
                      rec_in, F.perm.as<int32_t>(), F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
@@ -1425,11 +1622,11 @@ Status fmm_build(mf_plan *plan) {
   MF_TRY(need<int32_t>(F.nearF, static_cast<int64_t>(nleaf) * kNearF));
   MF_TRY(need<int32_t>(F.nearH, static_cast<int64_t>(nleaf) * kNearH));
   hipLaunchKernelGGL(near_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * kNearF, 256)),
-                     dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells,
+                     dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells, 3 * D,
                      tab.nearF.as<int3>(), kNearF, F.nearF.as<int32_t>());
   MF_LAUNCH_CHECK();
   hipLaunchKernelGGL(near_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * kNearH, 256)),
-                     dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells,
+                     dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells, 3 * D,
                      tab.nearH.as<int3>(), kNearH, F.nearH.as<int32_t>());
   MF_LAUNCH_CHECK();
   MF_TRY(need<int2>(F.rngF, static_cast<int64_t>(nleaf) * kNearF));
@@ -1444,6 +1641,19 @@ Status fmm_build(mf_plan *plan) {
                      F.nearH.as<int32_t>(), kNearH, kNearH, F.box_first.as<int32_t>(),
                      F.box_count.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>());
   MF_LAUNCH_CHECK();
+  {
+    // leaf shapes of the near-field kernels
+    MF_TRY(need<int32_t>(F.leaf_lists, 4 * static_cast<int64_t>(nleaf)));
+    void *pz = nullptr;
+    MF_TRY(workspace(ctx, "fmm_leaf_sizes", sizeof(int32_t) * 4, &pz));
+    MF_HIP(hipMemsetAsync(pz, 0, sizeof(int32_t) * 4, ctx->stream));
+    hipLaunchKernelGGL(classify_leaves_kernel, dim3(grid_for(nleaf, 256)), dim3(256), 0,
+                       ctx->stream, nleaf, F.box_count.as<int32_t>(), F.leaf_lists.as<int32_t>(),
+                       static_cast<int32_t *>(pz));
+    MF_LAUNCH_CHECK();
+    MF_HIP(hipMemcpyAsync(F.leaf_list_n, pz, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  }
   // 5. M2L source slots (levels >= 2) and the class-sorted tile lists
   MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox) * tab.nslotF));
   MF_TRY(need<int32_t>(F.srcH, static_cast<int64_t>(nleaf) * tab.nslotH));
@@ -1451,14 +1661,14 @@ Status fmm_build(mf_plan *plan) {
     if (cnt[l] == 0) continue;
     hipLaunchKernelGGL(m2l_list_kernel, dim3(grid_for(static_cast<int64_t>(cnt[l]) * tab.nslotF, 256)),
                        dim3(256), 0, ctx->stream, cnt[l], F.base[l],
-                       F.box_keys.as<uint64_t>() + F.base[l], 1 << l, tab.offsF.as<int3>(),
+                       F.box_keys.as<uint64_t>() + F.base[l], 1 << l, 3 * l, tab.offsF.as<int3>(),
                        tab.nslotF, F.srcF.as<int32_t>());
     MF_LAUNCH_CHECK();
   }
   if (D >= 2) {
     // leaf level Hessian slots: indexed by leaf (global index == leaf index, base[D] = 0)
     hipLaunchKernelGGL(m2l_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * tab.nslotH, 256)),
-                       dim3(256), 0, ctx->stream, nleaf, 0, F.box_keys.as<uint64_t>(), cells,
+                       dim3(256), 0, ctx->stream, nleaf, 0, F.box_keys.as<uint64_t>(), cells, 3 * D,
                        tab.offsH.as<int3>(), tab.nslotH, F.srcH.as<int32_t>());
     MF_LAUNCH_CHECK();
   }
@@ -1511,10 +1721,49 @@ Status fmm_build(mf_plan *plan) {
     MF_HIP(hipMemcpyAsync(F.tilesF.get(), tf.data(), sizeof(M2LTile) * tf.size(), hipMemcpyHostToDevice, ctx->stream));
     MF_HIP(hipMemcpyAsync(F.tilesH.get(), th.data(), sizeof(M2LTile) * th.size(), hipMemcpyHostToDevice, ctx->stream));
     MF_HIP(hipStreamSynchronize(ctx->stream));  // host vectors go out of scope
+    // non-empty offsets of every tile
+    const int ld = tab.slot_ld;
+    if (ld > 1024) return Status::Err(MF_ERR_UNSUPPORTED, "offset list longer than 1024");
+    MF_TRY(need<int32_t>(F.tlistF, static_cast<int64_t>(F.ntilesF) * ld));
+    MF_TRY(need<int32_t>(F.tlistH, static_cast<int64_t>(F.ntilesH) * ld));
+    MF_TRY(need<int32_t>(F.tnF, F.ntilesF));
+    MF_TRY(need<int32_t>(F.tnH, F.ntilesH));
+    auto tile_lists = [&](const M2LTile *tiles, int nt, const int32_t *src, int src_ld,
+                          int32_t *list, int32_t *nlist) -> Status {
+      if (nt <= 0) return Status::Ok();
+      hipLaunchKernelGGL(tile_slots_kernel, dim3(nt), dim3(256), 0, ctx->stream, tiles,
+                         F.class_ids2.as<int32_t>(), src, src_ld, tab.slot_tab.as<int32_t>(), ld,
+                         ld, list, nlist);
+      MF_LAUNCH_CHECK();
+      return Status::Ok();
+    };
+    MF_TRY(tile_lists(F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(), tab.nslotF,
+                      F.tlistF.as<int32_t>(), F.tnF.as<int32_t>()));
+    const int lt = F.leaf_tile_start;
+    MF_TRY(tile_lists(F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(), tab.nslotF,
+                      F.tlistH.as<int32_t>(), F.tnH.as<int32_t>()));
+    MF_TRY(tile_lists(F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt, F.srcH.as<int32_t>(), tab.nslotH,
+                      F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * ld,
+                      F.tnH.as<int32_t>() + lt));
   }
   MF_TRY(need<float>(F.Mt, static_cast<int64_t>(F.nbox) * F.KP));
   MF_TRY(need<float>(F.Lt, static_cast<int64_t>(F.nbox) * F.KP));
-  // work counts (one read-back)
+  F.counts_valid = false;  // work counts are computed on the first fmm_info
+  return Status::Ok();
+}
+
+Status fmm_counts(mf_plan *plan) {
+  mf_ctx *ctx = plan->ctx;
+  FmmState &F = *plan->fmm;
+  const FmmTables &tab = *F.tab;
+  const int D = F.D;
+  const std::vector<int> &cnt = F.count;
+  const int nleaf = F.nbox > 0 ? cnt[D] : 0;
+  if (nleaf == 0) {
+    F.p2p_field_pairs = F.p2p_force_pairs = F.m2l_pairs_field = F.m2l_pairs_force = 0;
+    F.counts_valid = true;
+    return Status::Ok();
+  }
   {
     void *pc = nullptr;
     MF_TRY(workspace(ctx, "fmm_counts", sizeof(unsigned long long) * 4, &pc));
@@ -1547,11 +1796,13 @@ Status fmm_build(mf_plan *plan) {
     F.m2l_pairs_field = static_cast<int64_t>(hc[2]);
     F.m2l_pairs_force = static_cast<int64_t>(hc[3]);
   }
+  F.counts_valid = true;
   return Status::Ok();
 }
 
-void fmm_info(const mf_plan *plan, mf_plan_info *info) {
-  const FmmState &F = *plan->fmm;
+Status fmm_info(mf_plan *plan, mf_plan_info *info) {
+  FmmState &F = *plan->fmm;
+  if (!F.counts_valid) MF_TRY(fmm_counts(plan));
   info->order = F.P;
   info->depth = F.D;
   info->leaves = F.count.empty() ? 0 : F.count[F.D];
@@ -1560,18 +1811,21 @@ void fmm_info(const mf_plan *plan, mf_plan_info *info) {
   info->p2p_force_pairs = F.p2p_force_pairs;
   info->m2l_field_pairs = F.m2l_pairs_field;
   info->m2l_force_pairs = F.m2l_pairs_force;
+  return Status::Ok();
 }
 
 namespace {
 
 template <int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
-                  int ntiles, const int32_t *src, int nslot) {
+                  int ntiles, const int32_t *src, int nslot, const int32_t *tlist,
+                  const int32_t *tn) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
 #define MF_M2L_ARGS                                                                           \
   ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tables, tiles, ntiles,                     \
-      F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld
+      F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
+      tn, tab.slot_ld
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
@@ -1609,8 +1863,8 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
     KernelTimer timer(ctx, "p2m");
     hipLaunchKernelGGL((p2m_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
-                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.lo[0], F.lo[1], F.lo[2],
-                       F.s_leaf, F.Mt.as<float>());
+                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(),
+                       Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, F.Mt.as<float>());
     MF_LAUNCH_CHECK();
   }
   {
@@ -1629,15 +1883,17 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
     const int prec = plan->m2l_precision();
     if (!hess) {
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
-                            tab.nslotF));
+                            tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>()));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
       // force lists srcH (indexed by leaf = global index), the others srcF.
       const int lt = F.leaf_tile_start;
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
-                            tab.nslotF));
+                            tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>()));
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
-                            F.srcH.as<int32_t>(), tab.nslotH));
+                            F.srcH.as<int32_t>(), tab.nslotH,
+                            F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * tab.slot_ld,
+                            F.tnH.as<int32_t>() + lt));
     }
   }
   {
@@ -1668,7 +1924,7 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   const int64_t n = F.n;
   if (n == 0) return Status::Ok();
   hipLaunchKernelGGL(pack_sorted_moments_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream,
-                     n, F.perm.as<int32_t>(), px, py, pz, scale, F.rec.as<SrcRec>());
+                     n, F.perm.as<int32_t>(), px, py, pz, scale, plan->weights, F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
   MF_TRY((upward_and_m2l<P, KP>(plan, false)));
   const int D = F.D, nleaf = F.count[D];
@@ -1680,16 +1936,26 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
     hipLaunchKernelGGL((l2p_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
-                       F.Lt.as<float>(), F.lo[0], F.lo[1], F.lo[2], F.s_leaf, h0, h1, h2, ox, oy,
-                       oz);
+                       F.Lt.as<float>(), Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, h0, h1, h2,
+                       ox, oy, oz);
     MF_LAUNCH_CHECK();
   }
   KernelTimer timer(ctx, "p2p_field");
-  hipLaunchKernelGGL(p2p_field_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.rec.as<SrcRec>(),
-                     F.perm.as<int32_t>(), F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, use_far,
-                     plan->fc, h0, h1, h2, ox, oy, oz);
-  MF_LAUNCH_CHECK();
+  const int32_t *lists = F.leaf_lists.as<int32_t>();
+#define MF_P2P_FIELD(T, k)                                                                       \
+  if (F.leaf_list_n[k] > 0) {                                                                    \
+    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(F.leaf_list_n[k]), dim3(64), 0, ctx->stream, \
+                       lists + static_cast<int64_t>(k) * nleaf, F.box_first.as<int32_t>(),       \
+                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
+                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, use_far, plan->fc, h0, h1,  \
+                       h2, ox, oy, oz);                                                          \
+    MF_LAUNCH_CHECK();                                                                           \
+  }
+  MF_P2P_FIELD(64, 0)
+  MF_P2P_FIELD(32, 1)
+  MF_P2P_FIELD(16, 2)
+  MF_P2P_FIELD(8, 3)
+#undef MF_P2P_FIELD
   return Status::Ok();
 }
 
@@ -1701,7 +1967,8 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
   const int64_t n = F.n;
   if (n == 0) return Status::Ok();
   hipLaunchKernelGGL(pack_sorted_moments_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream,
-                     n, F.perm.as<int32_t>(), mx, my, mz, 1.f, F.rec.as<SrcRec>());
+                     n, F.perm.as<int32_t>(), mx, my, mz, 1.f, static_cast<const float *>(nullptr),
+                     F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
   MF_TRY((upward_and_m2l<P, KP>(plan, true)));
   Text9 t{};
@@ -1719,16 +1986,25 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
     hipLaunchKernelGGL((l2p_force_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
-                       F.Lt.as<float>(), D >= 2 ? 1 : 0, F.lo[0], F.lo[1], F.lo[2], F.s_leaf, t,
-                       has_text, fx, fy, fz);
+                       F.Lt.as<float>(), D >= 2 ? 1 : 0, Frame{F.sys_lo.as<double>(), 3 * D},
+                       F.s_leaf, t, has_text, fx, fy, fz);
     MF_LAUNCH_CHECK();
   }
   KernelTimer timer(ctx, "p2p_force");
-  hipLaunchKernelGGL(p2p_force_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                     F.box_first.as<int32_t>(), F.box_count.as<int32_t>(), F.rec.as<SrcRec>(),
-                     F.perm.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc,
-                     fx, fy, fz);
-  MF_LAUNCH_CHECK();
+  const int32_t *lists = F.leaf_lists.as<int32_t>();
+#define MF_P2P_FORCE(T, k)                                                                       \
+  if (F.leaf_list_n[k] > 0) {                                                                    \
+    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(F.leaf_list_n[k]), dim3(64), 0, ctx->stream, \
+                       lists + static_cast<int64_t>(k) * nleaf, F.box_first.as<int32_t>(),       \
+                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
+                       F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, fx, fy, fz);     \
+    MF_LAUNCH_CHECK();                                                                           \
+  }
+  MF_P2P_FORCE(64, 0)
+  MF_P2P_FORCE(32, 1)
+  MF_P2P_FORCE(16, 2)
+  MF_P2P_FORCE(8, 3)
+#undef MF_P2P_FORCE
   return Status::Ok();
 }
 

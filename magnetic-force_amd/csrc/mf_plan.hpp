@@ -2,6 +2,7 @@
 #pragma once
 
 #include <memory>
+#include <vector>
 
 #include "mf_allpairs.hpp"
 #include "mf_internal.hpp"
@@ -32,6 +33,20 @@ struct mf_plan {
   // M2L precision of the FMM products; >= 0 overrides opts.m2l_precision
   // (the mixed-precision solve switches its inner products this way).
   int m2l_override = -1;
+  // Independent systems (batch plans): system s owns particles
+  // [sys_offsets[s], sys_offsets[s + 1]) in caller order; sys_of holds each
+  // particle's system (device int32, empty for a single system).
+  int nsys = 1;
+  std::vector<int64_t> sys_offsets;
+  mf::DeviceBuffer sys_of;
+  // Optional per-particle factor (caller order) applied to the moments of FMM
+  // field products; set by the batch solve for the span of its products.
+  const float *weights = nullptr;
+  // Segments of the per-system reductions (built on first batch solve):
+  // chunk c covers particles [seg[2c], seg[2c + 1]); system s owns chunks
+  // [seg_first[s], seg_first[s + 1]).
+  mf::DeviceBuffer seg, seg_first, kappa_part;
+  int nseg = 0;
   int m2l_precision() const { return m2l_override >= 0 ? m2l_override : opts.m2l_precision; }
 };
 
@@ -51,6 +66,10 @@ Status plan_pos64(mf_plan *plan, const double **px, const double **py, const dou
 
 Status solve(mf_plan *plan, double kappa, const float *bx, const float *by, const float *bz,
              float *xx, float *xy, float *xz, const mf_solve_opts &opts, mf_solve_stats *stats);
+// Every system s of a batch plan: (I - kappa[s] G_s) x = b (kappa: host, nsys).
+Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const float *by,
+                   const float *bz, float *xx, float *xy, float *xz, const mf_solve_opts &opts,
+                   mf_solve_stats *stats, double *sys_rel_residual);
 Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by,
                  const double *bz, double *xx, double *xy, double *xz,
                  const mf_solve_opts &opts, mf_solve_stats *stats);
@@ -61,6 +80,7 @@ Status fmm_apply_G(mf_plan *plan, float scale, const float *px, const float *py,
                    const float *hconst, float *ox, float *oy, float *oz);
 Status fmm_force(mf_plan *plan, const float *mx, const float *my, const float *mz,
                  const float *const *text9, float *fx, float *fy, float *fz);
-void fmm_info(const mf_plan *plan, mf_plan_info *info);
+// Geometry and work counts (the counts are computed on the first call after a build).
+Status fmm_info(mf_plan *plan, mf_plan_info *info);
 
 }  // namespace mf

@@ -700,4 +700,251 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
   return Status::Ok();
 }
 
+// ---------------------------------------------------------------------------
+// Batched CG over the independent systems of a batch plan: the CG recurrences
+// of every system run in lockstep, with per-system scalars (alpha, beta, r.r)
+// and one shared FMM product per iteration.  A system whose residual reaches
+// its tolerance is frozen (alpha = beta = 0) while the others continue.
+
+namespace {
+
+struct SegState {
+  double *bb, *rr, *rrn, *pap, *alpha, *beta, *partial, *maxratio;
+  int32_t *active;
+};
+
+// Per-chunk partial dot products: partial[c] = sum over chunk c of a . b.
+__global__ __launch_bounds__(kRedBlock) void seg_dot_kernel(const int64_t *__restrict__ seg, CV3 a,
+                                                            CV3 b, double *__restrict__ partial) {
+  __shared__ double sh[kRedBlock / 64];
+  const int64_t i0 = seg[2 * blockIdx.x], i1 = seg[2 * blockIdx.x + 1];
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kRedBlock)
+    acc += static_cast<double>(a.x[i]) * b.x[i] + static_cast<double>(a.y[i]) * b.y[i] +
+           static_cast<double>(a.z[i]) * b.z[i];
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[s] = sum of the partials of system s's chunks (one thread per system).
+__global__ void seg_sum_kernel(int nsys, const int32_t *__restrict__ seg_first,
+                               const double *__restrict__ partial, double *__restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  double acc = 0.0;
+  for (int c = seg_first[s]; c < seg_first[s + 1]; ++c) acc += partial[c];
+  out[s] = acc;
+}
+
+__global__ void fill_i32_kernel(int n, int32_t v, int32_t *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+__global__ void bcg_ap_kernel(int64_t n, CV3 p, CV3 g, V3 ap) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  ap.x[i] = p.x[i] - g.x[i];
+  ap.y[i] = p.y[i] - g.y[i];
+  ap.z[i] = p.z[i] - g.z[i];
+}
+
+__global__ void bcg_alpha_kernel(int nsys, SegState st) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsys) return;
+  st.alpha[s] = (st.active[s] && st.pap[s] != 0.0) ? st.rr[s] / st.pap[s] : 0.0;
+}
+
+__global__ void bcg_xr_kernel(int64_t n, const int32_t *__restrict__ sys_of,
+                              const double *__restrict__ alpha, CV3 p, CV3 ap, V3 x, V3 r) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float a = static_cast<float>(alpha[sys_of[i]]);
+  x.x[i] = fmaf(a, p.x[i], x.x[i]);
+  x.y[i] = fmaf(a, p.y[i], x.y[i]);
+  x.z[i] = fmaf(a, p.z[i], x.z[i]);
+  r.x[i] = fmaf(-a, ap.x[i], r.x[i]);
+  r.y[i] = fmaf(-a, ap.y[i], r.y[i]);
+  r.z[i] = fmaf(-a, ap.z[i], r.z[i]);
+}
+
+// beta, r.r update, activity and the largest relative residual over active
+// systems (one block).
+__global__ __launch_bounds__(kRedBlock) void bcg_beta_kernel(int nsys, double rtol, int first,
+                                                             SegState st) {
+  __shared__ double sh[kRedBlock];
+  double mx = 0.0;
+  for (int s = threadIdx.x; s < nsys; s += kRedBlock) {
+    const double bn = st.bb[s] > 0.0 ? sqrt(st.bb[s]) : 1.0;
+    if (first) {
+      st.rr[s] = st.rrn[s];
+      st.beta[s] = 0.0;
+    } else if (st.active[s]) {
+      st.beta[s] = st.rr[s] != 0.0 ? st.rrn[s] / st.rr[s] : 0.0;
+      st.rr[s] = st.rrn[s];
+    } else {
+      st.beta[s] = 0.0;
+    }
+    const double ratio = sqrt(st.rr[s]) / bn;
+    st.active[s] = (st.active[s] || first) && ratio > rtol ? 1 : 0;
+    mx = fmax(mx, ratio);
+  }
+  sh[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = kRedBlock / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st.maxratio[0] = sh[0];
+}
+
+__global__ void bcg_p_kernel(int64_t n, const int32_t *__restrict__ sys_of,
+                             const double *__restrict__ beta, CV3 r, V3 p) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float b = static_cast<float>(beta[sys_of[i]]);
+  p.x[i] = fmaf(b, p.x[i], r.x[i]);
+  p.y[i] = fmaf(b, p.y[i], r.y[i]);
+  p.z[i] = fmaf(b, p.z[i], r.z[i]);
+}
+
+Status build_segments(mf_plan *plan) {
+  if (plan->nseg > 0) return Status::Ok();
+  std::vector<int64_t> seg;
+  std::vector<int32_t> first;
+  constexpr int64_t kSeg = 16384;
+  for (int s = 0; s < plan->nsys; ++s) {
+    first.push_back(static_cast<int32_t>(seg.size() / 2));
+    for (int64_t b0 = plan->sys_offsets[s]; b0 < plan->sys_offsets[s + 1]; b0 += kSeg) {
+      seg.push_back(b0);
+      seg.push_back(std::min(plan->sys_offsets[s + 1], b0 + kSeg));
+    }
+  }
+  first.push_back(static_cast<int32_t>(seg.size() / 2));
+  const int nseg = static_cast<int>(seg.size() / 2);
+  MF_TRY(plan->seg.reserve(sizeof(int64_t) * std::max<size_t>(seg.size(), 2)));
+  MF_TRY(plan->seg_first.reserve(sizeof(int32_t) * first.size()));
+  mf_ctx *ctx = plan->ctx;
+  if (!seg.empty())
+    MF_HIP(hipMemcpyAsync(plan->seg.get(), seg.data(), sizeof(int64_t) * seg.size(),
+                          hipMemcpyHostToDevice, ctx->stream));
+  MF_HIP(hipMemcpyAsync(plan->seg_first.get(), first.data(), sizeof(int32_t) * first.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  MF_HIP(hipStreamSynchronize(ctx->stream));
+  plan->nseg = nseg;
+  return Status::Ok();
+}
+
+}  // namespace
+
+Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const float *by,
+                   const float *bz, float *xx, float *xy, float *xz, const mf_solve_opts &opts,
+                   mf_solve_stats *st, double *sys_rel_residual) {
+  *st = mf_solve_stats{};
+  mf_ctx *ctx = plan->ctx;
+  const int64_t n = plan->n;
+  const int nsys = plan->nsys;
+  if (n == 0) {
+    st->converged = 1;
+    for (int s = 0; s < nsys && sys_rel_residual; ++s) sys_rel_residual[s] = 0.0;
+    return Status::Ok();
+  }
+  MF_ARG(opts.method == MF_SOLVE_CG, "batch solves use CG");
+  const double rtol = opts.rtol > 0.0 ? opts.rtol : 1e-6;
+  const int maxit = opts.maxit > 0 ? opts.maxit : 1000;
+  MF_TRY(build_segments(plan));
+  // per-particle kappa (caller order), applied to the moments of every product
+  std::vector<float> kp(static_cast<size_t>(n));
+  for (int s = 0; s < nsys; ++s)
+    for (int64_t i = plan->sys_offsets[s]; i < plan->sys_offsets[s + 1]; ++i)
+      kp[static_cast<size_t>(i)] = static_cast<float>(kappa[s]);
+  MF_TRY(plan->kappa_part.reserve(sizeof(float) * static_cast<size_t>(n)));
+  MF_HIP(hipMemcpyAsync(plan->kappa_part.get(), kp.data(), sizeof(float) * kp.size(),
+                        hipMemcpyHostToDevice, ctx->stream));
+  Work w{};
+  MF_TRY(get_work(ctx, n, w));
+  void *pv = nullptr;
+  MF_TRY(workspace(ctx, "batch_scalars", sizeof(double) * (7 * static_cast<size_t>(nsys) + plan->nseg + 1) +
+                                             sizeof(int32_t) * static_cast<size_t>(nsys), &pv));
+  SegState S{};
+  double *d = static_cast<double *>(pv);
+  S.bb = d;
+  S.rr = d + nsys;
+  S.rrn = d + 2 * nsys;
+  S.pap = d + 3 * nsys;
+  S.alpha = d + 4 * nsys;
+  S.beta = d + 5 * nsys;
+  S.maxratio = d + 6 * nsys;
+  S.partial = d + 7 * nsys;
+  S.active = reinterpret_cast<int32_t *>(d + 7 * nsys + plan->nseg + 1);
+  const int32_t *sys_of = plan->sys_of.as<int32_t>();
+  const int64_t *seg = plan->seg.as<int64_t>();
+  const int32_t *seg_first = plan->seg_first.as<int32_t>();
+  const int nseg = plan->nseg;
+  const CV3 b{bx, by, bz};
+  V3 x{xx, xy, xz};
+  auto seg_dot = [&](CV3 a, CV3 c, double *out) -> Status {
+    hipLaunchKernelGGL(seg_dot_kernel, dim3(nseg), dim3(kRedBlock), 0, ctx->stream, seg, a, c,
+                       S.partial);
+    MF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(seg_sum_kernel, dim3((nsys + 255) / 256), dim3(256), 0, ctx->stream, nsys,
+                       seg_first, static_cast<const double *>(S.partial), out);
+    MF_LAUNCH_CHECK();
+    return Status::Ok();
+  };
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((nsys + 255) / 256), dim3(256), 0, ctx->stream, nsys, 0,
+                     S.active);
+  MF_LAUNCH_CHECK();
+  MF_TRY(seg_dot(b, b, S.bb));
+  // products with per-particle kappa for the span of the solve
+  struct WeightGuard {
+    mf_plan *p;
+    ~WeightGuard() { p->weights = nullptr; }
+  } guard{plan};
+  plan->weights = plan->kappa_part.as<float>();
+  MF_TRY(residual(plan, 1.0, b, x, w, true, opts.zero_start != 0));
+  if (!opts.zero_start) st->matvecs += 1;
+  const CV3 r = cv3(w.r, n), p = cv3(w.p, n), ap = cv3(w.ap, n), g = cv3(w.g, n);
+  MF_TRY(seg_dot(r, r, S.rrn));
+  hipLaunchKernelGGL(bcg_beta_kernel, dim3(1), dim3(kRedBlock), 0, ctx->stream, nsys, rtol, 1, S);
+  MF_LAUNCH_CHECK();
+  double mx = 0.0;
+  MF_TRY(read_scalar(ctx, S.maxratio, &mx));
+  int it = 0;
+  while (mx > rtol && st->matvecs < maxit) {
+    MF_TRY(plan_apply_G(plan, 1.f, w.p, w.p + n, w.p + 2 * n, nullptr, w.g, w.g + n, w.g + 2 * n));
+    st->matvecs += 1;
+    hipLaunchKernelGGL(bcg_ap_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, p, g,
+                       v3(w.ap, n));
+    MF_LAUNCH_CHECK();
+    MF_TRY(seg_dot(p, ap, S.pap));
+    hipLaunchKernelGGL(bcg_alpha_kernel, dim3((nsys + 255) / 256), dim3(256), 0, ctx->stream, nsys, S);
+    MF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bcg_xr_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, sys_of,
+                       static_cast<const double *>(S.alpha), p, ap, x, v3(w.r, n));
+    MF_LAUNCH_CHECK();
+    MF_TRY(seg_dot(r, r, S.rrn));
+    hipLaunchKernelGGL(bcg_beta_kernel, dim3(1), dim3(kRedBlock), 0, ctx->stream, nsys, rtol, 0, S);
+    MF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bcg_p_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, sys_of,
+                       static_cast<const double *>(S.beta), r, v3(w.p, n));
+    MF_LAUNCH_CHECK();
+    ++it;
+    MF_TRY(read_scalar(ctx, S.maxratio, &mx));
+    if (!std::isfinite(mx)) return Status::Err(MF_ERR_NOT_CONVERGED, "CG residual is not finite");
+  }
+  st->iterations = it;
+  st->rel_residual = mx;
+  st->converged = mx <= rtol;
+  if (sys_rel_residual) {
+    std::vector<double> rr(nsys), bb(nsys);
+    MF_HIP(hipMemcpyAsync(rr.data(), S.rr, sizeof(double) * nsys, hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipMemcpyAsync(bb.data(), S.bb, sizeof(double) * nsys, hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+    for (int s = 0; s < nsys; ++s)
+      sys_rel_residual[s] = std::sqrt(rr[s]) / (bb[s] > 0.0 ? std::sqrt(bb[s]) : 1.0);
+  }
+  return Status::Ok();
+}
+
 }  // namespace mf

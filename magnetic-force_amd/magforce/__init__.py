@@ -128,6 +128,11 @@ _SIGS = {
                        ctypes.POINTER(SolveStats)], _i),
     "mf_plan_solve_f64": ([_vp, _d, _vp, _vp, ctypes.POINTER(SolveOpts),
                            ctypes.POINTER(SolveStats)], _i),
+    "mf_plan_create_batch": ([_vp, _i, ctypes.POINTER(_i64), _vp, _vp, _vp, _f,
+                              ctypes.POINTER(PlanOpts), ctypes.POINTER(_vp)], _i),
+    "mf_plan_solve_batch": ([_vp, ctypes.POINTER(_d), _vp, _vp, _vp, _vp, _vp, _vp,
+                             ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveStats),
+                             ctypes.POINTER(_d)], _i),
     "mf_vec_axpby": ([_vp, _i64, _f, _vp, _vp, _vp, _f, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mf_vec_mul": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mf_vec_dot": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_d)], _i),
@@ -433,6 +438,50 @@ class Plan:
         _check(lib().mf_plan_solve_f64(self.h_plan, float(kappa), _ptr(bb), _ptr(x),
                                        ctypes.byref(opts), ctypes.byref(st)), self.ctx.h)
         return SolveResult(x, st.as_dict())
+
+
+class BatchPlan(Plan):
+    """Independent particle systems in one FMM plan (mf_plan_create_batch):
+    system s owns columns offsets[s]:offsets[s+1] of pos.  field/force are
+    block-diagonal; solve() runs the batched CG with one kappa per system."""
+
+    def __init__(self, ctx: Context, pos, offsets, h: float, order: int = 0,
+                 m2l_precision: int = 0):
+        self.ctx = ctx
+        self.h = float(h)
+        p = _soa(pos)
+        self.n = p.shape[1]
+        off = [int(v) for v in offsets]
+        if off[0] != 0 or off[-1] != self.n:
+            raise ValueError("offsets must start at 0 and end at n")
+        self.offsets = off
+        self.nsys = len(off) - 1
+        self.opts = PlanOpts(mode=MODE_FMM, order=order, m2l_precision=m2l_precision)
+        arr = (ctypes.c_int64 * len(off))(*off)
+        hp = ctypes.c_void_p()
+        _check(lib().mf_plan_create_batch(ctx.h, self.nsys, arr, _ptr(p, 0), _ptr(p, 1),
+                                          _ptr(p, 2), ctypes.c_float(self.h),
+                                          ctypes.byref(self.opts), ctypes.byref(hp)), ctx.h)
+        self.h_plan = hp
+        self.pos = p
+
+    def solve(self, kappas, b, x0=None, rtol: float = 1e-6, maxit: int = 1000):
+        import torch
+        bb = _soa(b, self.n)
+        if len(kappas) != self.nsys:
+            raise ValueError("one kappa per system")
+        x = torch.empty_like(bb) if x0 is None else _soa(x0, self.n).clone()
+        opts = SolveOpts(method=SOLVE_CG, maxit=maxit, rtol=rtol,
+                         zero_start=1 if x0 is None else 0)
+        st = SolveStats()
+        kap = (ctypes.c_double * self.nsys)(*[float(k) for k in kappas])
+        res = (ctypes.c_double * self.nsys)()
+        _check(lib().mf_plan_solve_batch(self.h_plan, kap, _ptr(bb, 0), _ptr(bb, 1), _ptr(bb, 2),
+                                         _ptr(x, 0), _ptr(x, 1), _ptr(x, 2), ctypes.byref(opts),
+                                         ctypes.byref(st), res), self.ctx.h)
+        d = st.as_dict()
+        d["sys_rel_residual"] = list(res)
+        return SolveResult(x, d)
 
 
 def uniform_field(n: int, hconst, device="cuda"):

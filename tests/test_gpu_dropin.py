@@ -41,3 +41,21 @@ def test_example_naive_mode(tmp_path, oracle):
     # force at the exported state (external tensor carries mu0 once)
     F = oracle.force_allpairs(pos, kappa * x, 1.0, text=g["text"])
     assert rel_err(force[:, 4:7], F) < 1e-4
+
+
+def test_example_fmm_mode(tmp_path, oracle):
+    """The same sequence with the FMM evaluators (the reference's default
+    mode): the 2x2x2 lattice fits in one leaf, so every product is the exact
+    near field and the exported state matches the naive run."""
+    _, data, force = run_example(tmp_path, "fmm")
+    naive_dir = tmp_path / "naive"
+    naive_dir.mkdir()
+    _, data_n, force_n = run_example(naive_dir, "naive")
+    g = np.load(os.path.join(GOLD, "example.npz"))
+    assert data.shape == (4, 10) and force.shape == (4, 10)
+    pos, x, hext = data[:, 1:4], data[:, 4:7], data[:, 7:10]
+    assert rel_err(hext, g["hext"]) < 1e-5
+    assert rel_err(x, data_n[:, 4:7]) < 1e-4
+    kappa = 0.5
+    F = oracle.force_allpairs(pos, kappa * x, 1.0, text=g["text"])
+    assert rel_err(force[:, 4:7], F) < 1e-4
