@@ -457,39 +457,63 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t (&p)[NPART]) {
   }
 }
 
-constexpr int kGatherIt = kTileT / 8;  // 64 * KP / 4 / (2 KP)
+// Source multipoles of one offset for the tile: the 64 x KP block as 16-byte
+// chunks, FP32 float4 (NPART = 0) or BF16x8 of the pre-split parts (Mb,
+// [box][3][KP]).  Chunk e (e = tid + it * 2 KP) is target e / PER_T, chunk
+// e % PER_T of that target.
+template <int KP, int NPART>
+struct M2LGather {
+  static constexpr int PER_T = NPART == 0 ? KP / 4 : NPART * KP / 8;  // chunks per target
+  static constexpr int IT = 64 * PER_T / (2 * KP);                    // chunks per thread
+};
 
-
-template <int KP>
+template <int KP, int NPART>
 __device__ __forceinline__ void m2l_src_load(const int *s_tgt, const int32_t *__restrict__ src_slots,
                                              int src_ld, int slot, int tid,
-                                             int (&src)[kGatherIt]) {
+                                             int (&src)[M2LGather<KP, NPART>::IT]) {
+  using G = M2LGather<KP, NPART>;
 #pragma unroll
-  for (int it = 0; it < kGatherIt; ++it) {
-    const int tgt = s_tgt[(tid + it * 2 * KP) / (KP / 4)];
+  for (int it = 0; it < G::IT; ++it) {
+    const int tgt = s_tgt[(tid + it * 2 * KP) / G::PER_T];
     src[it] = tgt >= 0 ? src_slots[static_cast<int64_t>(tgt) * src_ld + slot] : -1;
   }
 }
 
-template <int KP>
-__device__ __forceinline__ int m2l_mult_load(const float *__restrict__ Mt,
-                                             const int (&src)[kGatherIt], int tid,
-                                             float4 (&v)[kGatherIt]) {
+template <int KP, int NPART>
+__device__ __forceinline__ int m2l_mult_load(const void *__restrict__ Msrc,
+                                             const int (&src)[M2LGather<KP, NPART>::IT], int tid,
+                                             float4 (&v)[M2LGather<KP, NPART>::IT]) {
+  using G = M2LGather<KP, NPART>;
   int any = 0;
 #pragma unroll
-  for (int it = 0; it < kGatherIt; ++it) {
-    const int q = (tid + it * 2 * KP) % (KP / 4);
+  for (int it = 0; it < G::IT; ++it) {
+    const int r = (tid + it * 2 * KP) % G::PER_T;
     v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (src[it] >= 0) {
-      v[it] = reinterpret_cast<const float4 *>(Mt + static_cast<int64_t>(src[it]) * KP)[q];
+      const float4 *base = reinterpret_cast<const float4 *>(Msrc) +
+                          static_cast<int64_t>(src[it]) * (NPART == 0 ? KP / 4 : 3 * KP / 8);
+      v[it] = base[r];  // BF16: chunk r of the box's [3][KP] parts block
       any = 1;
     }
   }
   return any;
 }
 
+// Pre-split multipoles: Mb[(box * 3 + p) * KP + d] = BF16 part p of Mt[box * KP + d].
+__global__ void split_multipoles_kernel(int64_t nelem, int KP, const float *__restrict__ Mt,
+                                        uint16_t *__restrict__ Mb) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nelem) return;
+  const int64_t box = i / KP;
+  const int d = static_cast<int>(i % KP);
+  uint32_t p[3];
+  split_bf16<3>(Mt[i], p);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) Mb[(box * 3 + k) * KP + d] = static_cast<uint16_t>(p[k]);
+}
+
 template <int KP, int NPART>
-__global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ Mt,
+__global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Msrc,
                                                           float *__restrict__ Lt,
                                                           const void *__restrict__ Tv,
                                                           const M2LTile *__restrict__ tiles,
@@ -528,41 +552,35 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const float *__restrict__ M
   // the tile's non-empty offsets (plan build), walked in slot order
   const int32_t *__restrict__ tl = tile_list + static_cast<int64_t>(tile_id) * list_ld;
   const int na = tile_nlist[tile_id];
-  int src[kGatherIt];
-  float4 v[kGatherIt];
+  using GA = M2LGather<KP, NPART>;
+  int src[GA::IT];
+  float4 v[GA::IT];
   int any = 0;
   if (na > 0) {
-    m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[0], tid, src);
-    any = m2l_mult_load<KP>(Mt, src, tid, v);
-    if (na > 1) m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[1], tid, src);
+    m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[0], tid, src);
+    any = m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
+    if (na > 1) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[1], tid, src);
   }
   for (int pc = 0; pc < na; ++pc) {
     __syncthreads();  // the previous offset's operands are consumed
 #pragma unroll
-    for (int it = 0; it < kGatherIt; ++it) {
+    for (int it = 0; it < GA::IT; ++it) {
       const int e = tid + it * NT;
-      const int j = e / (KP / 4), q = e % (KP / 4);
+      const int j = e / GA::PER_T, r = e % GA::PER_T;
       if constexpr (NPART == 0) {
         float *sF = reinterpret_cast<float *>(s_raw);
-        *reinterpret_cast<float4 *>(&sF[j * LDF + 4 * q]) = v[it];
+        *reinterpret_cast<float4 *>(&sF[j * LDF + 4 * r]) = v[it];
       } else {
         uint16_t *sB = reinterpret_cast<uint16_t *>(s_raw);
-        uint32_t px[NPART], py[NPART], pz[NPART], pw[NPART];
-        split_bf16<NPART>(v[it].x, px);
-        split_bf16<NPART>(v[it].y, py);
-        split_bf16<NPART>(v[it].z, pz);
-        split_bf16<NPART>(v[it].w, pw);
-#pragma unroll
-        for (int p = 0; p < NPART; ++p)
-          *reinterpret_cast<uint2 *>(&sB[(p * kTileT + j) * LDB + 4 * q]) =
-              make_uint2(px[p] | (py[p] << 16), pz[p] | (pw[p] << 16));
+        const int p = r / (KP / 8), q8 = r % (KP / 8);
+        *reinterpret_cast<float4 *>(&sB[(p * kTileT + j) * LDB + 8 * q8]) = v[it];
       }
     }
     const int go = __syncthreads_or(any);
     const int tix = smap[tl[pc]];
     if (pc + 1 < na) {
-      any = m2l_mult_load<KP>(Mt, src, tid, v);
-      if (pc + 2 < na) m2l_src_load<KP>(s_tgt, src_slots, src_ld, tl[pc + 2], tid, src);
+      any = m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
+      if (pc + 2 < na) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[pc + 2], tid, src);
     }
     if (go) {
       if constexpr (NPART == 0) {
@@ -1306,6 +1324,7 @@ struct FmmState {
   int ntilesF = 0, ntilesH = 0;
   int leaf_tile_start = 0;  // first tile of the leaf level (tiles are in level order)
   DeviceBuffer Mt, Lt;
+  DeviceBuffer Mb;  // BF16 parts of Mt [nbox][3][KP], for the BF16-split M2L
   int64_t p2p_field_pairs = 0;  // sum over leaves of targets x near sources
   int64_t p2p_force_pairs = 0;
   int64_t m2l_pairs_field = 0;
@@ -1748,6 +1767,7 @@ Status fmm_build(mf_plan *plan) {
   }
   MF_TRY(need<float>(F.Mt, static_cast<int64_t>(F.nbox) * F.KP));
   MF_TRY(need<float>(F.Lt, static_cast<int64_t>(F.nbox) * F.KP));
+  MF_TRY(need<uint16_t>(F.Mb, static_cast<int64_t>(F.nbox) * 3 * F.KP));
   F.counts_valid = false;  // work counts are computed on the first fmm_info
   return Status::Ok();
 }
@@ -1823,22 +1843,25 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
 #define MF_M2L_ARGS                                                                           \
-  ctx->stream, F.Mt.as<float>(), F.Lt.as<float>(), tables, tiles, ntiles,                     \
+  ctx->stream, msrc, F.Lt.as<float>(), tables, tiles, ntiles,                                 \
       F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
       tn, tab.slot_ld
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
+      const void *msrc = F.Mt.get();
       hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     case MF_M2L_BF16X6: {
       const void *tables = tab.Tb.get();
+      const void *msrc = F.Mb.get();
       hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     case MF_M2L_BF16X3: {
       const void *tables = tab.Tb.get();
+      const void *msrc = F.Mb.get();
       hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
@@ -1881,6 +1904,13 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
   {
     KernelTimer timer(ctx, hess ? "m2l_force" : "m2l");
     const int prec = plan->m2l_precision();
+    if (prec != MF_M2L_FP32) {
+      // split every multipole once; the M2L gathers each one many times
+      const int64_t ne = static_cast<int64_t>(F.nbox) * KP;
+      hipLaunchKernelGGL(split_multipoles_kernel, dim3(grid_for(ne, 256)), dim3(256), 0,
+                         ctx->stream, ne, KP, F.Mt.as<float>(), F.Mb.as<uint16_t>());
+      MF_LAUNCH_CHECK();
+    }
     if (!hess) {
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
                             tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>()));
