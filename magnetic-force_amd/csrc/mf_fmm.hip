@@ -334,67 +334,6 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
   }
 }
 
-// Dense per-octant translation on one level: out[box] (+)= W[oct] in[other].
-// mode 0 (M2M): out = parent, loop over its children (in = child Mt).
-// mode 1 (L2L): out = child, in = parent Lt, accumulate.
-template <int KP>
-__global__ __launch_bounds__(64) void m2m_kernel(int nparent, int parent_base,
-                                                 const int32_t *__restrict__ first_child,
-                                                 const int32_t *__restrict__ nchild,
-                                                 const uint64_t *__restrict__ child_keys,
-                                                 int child_base, const float *__restrict__ W,
-                                                 int ndof, float *__restrict__ Mt) {
-  const int pb = blockIdx.x;
-  if (pb >= nparent) return;
-  const int lane = threadIdx.x;
-  constexpr int R = (KP + 63) / 64;
-  float acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.f;
-  const int c0 = first_child[pb], nc = nchild[pb];
-  for (int ci = 0; ci < nc; ++ci) {
-    const int c = c0 + ci;  // level-local child index
-    const int oct = static_cast<int>(child_keys[c] & 7ULL);
-    const float *__restrict__ w = W + static_cast<int64_t>(oct) * KP * KP;
-    const float *__restrict__ m = Mt + static_cast<int64_t>(child_base + c) * KP;
-    for (int col = 0; col < ndof; ++col) {
-      const float mv = m[col];
-#pragma unroll
-      for (int r = 0; r < R; ++r) acc[r] = fmaf(w[col * KP + min(lane + 64 * r, KP - 1)], mv, acc[r]);
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (lane + 64 * r < KP) Mt[static_cast<int64_t>(parent_base + pb) * KP + lane + 64 * r] = acc[r];
-}
-
-template <int KP>
-__global__ __launch_bounds__(64) void l2l_kernel(int nchild, int child_base,
-                                                 const int32_t *__restrict__ parent,
-                                                 const uint64_t *__restrict__ child_keys,
-                                                 const float *__restrict__ W, int ndof,
-                                                 float *__restrict__ Lt) {
-  const int c = blockIdx.x;
-  if (c >= nchild) return;
-  const int lane = threadIdx.x;
-  constexpr int R = (KP + 63) / 64;
-  const int oct = static_cast<int>(child_keys[c] & 7ULL);
-  const float *__restrict__ w = W + static_cast<int64_t>(oct) * KP * KP;
-  const float *__restrict__ l = Lt + static_cast<int64_t>(parent[c]) * KP;
-  float acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.f;
-  for (int col = 0; col < ndof; ++col) {
-    const float lv = l[col];
-#pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = fmaf(w[col * KP + min(lane + 64 * r, KP - 1)], lv, acc[r]);
-  }
-  float *__restrict__ out = Lt + static_cast<int64_t>(child_base + c) * KP;
-#pragma unroll
-  for (int r = 0; r < R; ++r)
-    if (lane + 64 * r < KP) out[lane + 64 * r] += acc[r];
-}
-
 // ---------------------------------------------------------------------------
 // M2L on MFMA.  A workgroup owns a tile of up to kTileT = 64 target boxes of
 // one (level, parity) class and walks that class's offset list.  Per offset
@@ -432,6 +371,7 @@ __global__ __launch_bounds__(64) void l2l_kernel(int nchild, int child_base,
 //    consecutive k of one row are one 16-byte load; the gathered multipoles
 //    are split as they are staged into LDS.
 constexpr int kTileT = 64;
+constexpr int kRowM2M = 16, kRowL2L = 17, kSlotRows = 25;  // slot-table rows
 
 struct M2LTile {
   int first;   // position in the class-sorted box list
@@ -525,7 +465,7 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
                                                           int slot_ld,
                                                           const int32_t *__restrict__ tile_list,
                                                           const int32_t *__restrict__ tile_nlist,
-                                                          int list_ld) {
+                                                          int list_ld, int accumulate) {
   constexpr int NT = 2 * KP;
   constexpr int LDF = KP + 4;  // FP32 row pitch (floats)
   constexpr int LDB = KP + 8;  // BF16 row pitch (elements)
@@ -644,7 +584,8 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = 32 * wave + 16 * rb + 4 * kq + r;
-        Lt[static_cast<int64_t>(tgt) * KP + row] = acc[rb][cb][r];
+        float *o = Lt + static_cast<int64_t>(tgt) * KP + row;
+        *o = accumulate ? *o + acc[rb][cb][r] : acc[rb][cb][r];
       }
   }
 }
@@ -677,6 +618,19 @@ __global__ __launch_bounds__(256) void tile_slots_kernel(const M2LTile *__restri
       if (s_flag[k] && smap[k] >= 0) list[static_cast<int64_t>(blockIdx.x) * list_ld + c++] = k;
     nlist[blockIdx.x] = c;
   }
+}
+
+// child_tab[parent * 8 + octant] = global index of the child (one level).
+__global__ void child_table_kernel(int nchild, int child_base, const int32_t *__restrict__ parent,
+                                   const uint64_t *__restrict__ keys, int32_t *__restrict__ child_tab) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nchild) return;
+  child_tab[static_cast<int64_t>(parent[c]) * 8 + static_cast<int>(keys[c] & 7ULL)] = child_base + c;
+}
+
+__global__ void iota_kernel(int n, int32_t *__restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1287,7 +1241,7 @@ struct FmmTables {
   DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP rows][KP] row-major
   DeviceBuffer W_m2m;     // [8][KP][KP]
   DeviceBuffer W_l2l;     // [8][KP][KP]
-  DeviceBuffer slot_tab;  // [2][8][nslot_max] table index per slot (-1 unused)
+  DeviceBuffer slot_tab;  // [kSlotRows][slot_ld] table index per slot (-1 unused)
   DeviceBuffer offsF;     // [8][nslotF] int3 (padding 0x7fffffff)
   DeviceBuffer offsH;     // [8][nslotH]
   DeviceBuffer nearF;     // [81] int3
@@ -1323,6 +1277,13 @@ struct FmmState {
   DeviceBuffer tlistF, tlistH, tnF, tnH;  // non-empty offsets per tile [ntiles][slot_ld], counts
   int ntilesF = 0, ntilesH = 0;
   int leaf_tile_start = 0;  // first tile of the leaf level (tiles are in level order)
+  // Upward and downward translations on the M2L kernel: M2M tiles (parents,
+  // slots = octants of the children) and L2L tiles (children, one slot: the
+  // parent), in level order with per-level start indices.
+  DeviceBuffer child_tab;  // int32 [nbox][8]: global child index per octant, -1 none
+  DeviceBuffer box_ids;    // int32 [nbox]: identity list (M2M tile targets)
+  DeviceBuffer tilesM, tilesL, tlistM, tnM, tlistL, tnL;
+  std::vector<int> tileM_start, tileL_start;  // [level], size D + 2
   DeviceBuffer Mt, Lt;
   DeviceBuffer Mb;  // BF16 parts of Mt [nbox][3][KP], for the BF16-split M2L
   int64_t p2p_field_pairs = 0;  // sum over leaves of targets x near sources
@@ -1381,7 +1342,13 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
         }
       }
   t->slot_ld = std::max(t->nslotF, t->nslotH);
-  std::vector<int32_t> slots(2 * 8 * t->slot_ld, -1);
+  // rows: 0..7 field M2L per parity, 8..15 force leaf M2L, 16 M2M (slot k ->
+  // octant k), 17..24 L2L of a child of octant k (slot 0 -> octant k)
+  std::vector<int32_t> slots(kSlotRows * t->slot_ld, -1);
+  for (int k = 0; k < 8; ++k) {
+    slots[kRowM2M * t->slot_ld + k] = k;
+    slots[(kRowL2L + k) * t->slot_ld] = k;
+  }
   std::vector<int3> hof(8 * t->nslotF, make_int3(0x7fffffff, 0, 0));
   std::vector<int3> hoh(8 * t->nslotH, make_int3(0x7fffffff, 0, 0));
   for (int b = 0; b < 8; ++b) {
@@ -1764,6 +1731,78 @@ Status fmm_build(mf_plan *plan) {
     MF_TRY(tile_lists(F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt, F.srcH.as<int32_t>(), tab.nslotH,
                       F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * ld,
                       F.tnH.as<int32_t>() + lt));
+    // M2M / L2L tiles
+    MF_TRY(need<int32_t>(F.child_tab, static_cast<int64_t>(F.nbox) * 8));
+    MF_TRY(need<int32_t>(F.box_ids, F.nbox));
+    MF_HIP(hipMemsetAsync(F.child_tab.get(), 0xff, sizeof(int32_t) * 8 * static_cast<size_t>(F.nbox),
+                          ctx->stream));
+    hipLaunchKernelGGL(iota_kernel, dim3(grid_for(F.nbox, 256)), dim3(256), 0, ctx->stream, F.nbox,
+                       F.box_ids.as<int32_t>());
+    MF_LAUNCH_CHECK();
+    for (int l = 3; l <= D; ++l) {
+      hipLaunchKernelGGL(child_table_kernel, dim3(grid_for(cnt[l], 256)), dim3(256), 0, ctx->stream,
+                         cnt[l], F.base[l], F.box_parent.as<int32_t>() + F.base[l],
+                         F.box_keys.as<uint64_t>() + F.base[l], F.child_tab.as<int32_t>());
+      MF_LAUNCH_CHECK();
+    }
+    std::vector<M2LTile> tm, tlv;
+    F.tileM_start.assign(D + 2, 0);
+    F.tileL_start.assign(D + 2, 0);
+    for (int l = 2; l <= D; ++l) {  // M2M targets: parents at levels 2 .. D - 1
+      F.tileM_start[l] = static_cast<int>(tm.size());
+      if (l < D)
+        for (int s = 0; s < cnt[l]; s += kTileT)
+          tm.push_back(M2LTile{F.base[l] + s, std::min(kTileT, cnt[l] - s), kRowM2M, 8});
+    }
+    F.tileM_start[D + 1] = static_cast<int>(tm.size());
+    {
+      int level = 2, pos_in = 0;  // walk the field tiles (level order) to find level bounds
+      std::vector<int> tile_level(tf.size());
+      int acc_boxes = 0;
+      for (size_t k = 0; k < tf.size(); ++k) {
+        while (level <= D && tf[k].first >= acc_boxes + cnt[level]) {
+          acc_boxes += cnt[level];
+          ++level;
+        }
+        tile_level[k] = level;
+      }
+      (void)pos_in;
+      for (int l = 2; l <= D + 1; ++l) F.tileL_start[l] = 0;
+      for (int l = 3; l <= D; ++l) {  // L2L targets: children at levels 3 .. D
+        F.tileL_start[l] = static_cast<int>(tlv.size());
+        for (size_t k = 0; k < tf.size(); ++k)
+          if (tile_level[k] == l)
+            tlv.push_back(M2LTile{tf[k].first, tf[k].count, kRowL2L + tf[k].slots, 1});
+      }
+      F.tileL_start[D + 1] = static_cast<int>(tlv.size());
+    }
+    MF_TRY(need<M2LTile>(F.tilesM, static_cast<int64_t>(tm.size())));
+    MF_TRY(need<M2LTile>(F.tilesL, static_cast<int64_t>(tlv.size())));
+    MF_TRY(need<int32_t>(F.tlistM, static_cast<int64_t>(tm.size()) * ld));
+    MF_TRY(need<int32_t>(F.tnM, static_cast<int64_t>(tm.size())));
+    MF_TRY(need<int32_t>(F.tlistL, static_cast<int64_t>(tlv.size()) * ld));
+    MF_TRY(need<int32_t>(F.tnL, static_cast<int64_t>(tlv.size())));
+    if (!tm.empty())
+      MF_HIP(hipMemcpyAsync(F.tilesM.get(), tm.data(), sizeof(M2LTile) * tm.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    if (!tlv.empty())
+      MF_HIP(hipMemcpyAsync(F.tilesL.get(), tlv.data(), sizeof(M2LTile) * tlv.size(),
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (!tm.empty()) {
+      hipLaunchKernelGGL(tile_slots_kernel, dim3(static_cast<int>(tm.size())), dim3(256), 0, ctx->stream,
+                         F.tilesM.as<M2LTile>(), F.box_ids.as<int32_t>(), F.child_tab.as<int32_t>(),
+                         8, tab.slot_tab.as<int32_t>(), ld, ld, F.tlistM.as<int32_t>(),
+                         F.tnM.as<int32_t>());
+      MF_LAUNCH_CHECK();
+    }
+    if (!tlv.empty()) {
+      hipLaunchKernelGGL(tile_slots_kernel, dim3(static_cast<int>(tlv.size())), dim3(256), 0,
+                         ctx->stream, F.tilesL.as<M2LTile>(), F.class_ids2.as<int32_t>(),
+                         F.box_parent.as<int32_t>(), 1, tab.slot_tab.as<int32_t>(), ld, ld,
+                         F.tlistL.as<int32_t>(), F.tnL.as<int32_t>());
+      MF_LAUNCH_CHECK();
+    }
+    MF_HIP(hipStreamSynchronize(ctx->stream));  // host tile vectors go out of scope
   }
   MF_TRY(need<float>(F.Mt, static_cast<int64_t>(F.nbox) * F.KP));
   MF_TRY(need<float>(F.Lt, static_cast<int64_t>(F.nbox) * F.KP));
@@ -1845,7 +1884,7 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
 #define MF_M2L_ARGS                                                                           \
   ctx->stream, msrc, F.Lt.as<float>(), tables, tiles, ntiles,                                 \
       F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
-      tn, tab.slot_ld
+      tn, tab.slot_ld, 0
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
@@ -1881,7 +1920,6 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
   const int D = F.D;
   const int nleaf = F.count[D];
   if (D < 2) return Status::Ok();
-  constexpr int ND = (P + 1) * (P + 1);
   {
     KernelTimer timer(ctx, "p2m");
     hipLaunchKernelGGL((p2m_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
@@ -1891,13 +1929,18 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
     MF_LAUNCH_CHECK();
   }
   {
+    // M2M on the M2L kernel (FP32 MFMA): parents of level l gather their
+    // children's multipoles per octant and apply that octant's matrix.
     KernelTimer timer(ctx, "m2m");
     for (int l = D - 1; l >= 2; --l) {
-      hipLaunchKernelGGL((m2m_kernel<KP>), dim3(F.count[l]), dim3(64), 0, ctx->stream, F.count[l],
-                         F.base[l], F.box_first.as<int32_t>() + F.base[l],
-                         F.box_count.as<int32_t>() + F.base[l],
-                         F.box_keys.as<uint64_t>() + F.base[l + 1], F.base[l + 1],
-                         tab.W_m2m.as<float>(), ND, F.Mt.as<float>());
+      const int t0 = F.tileM_start[l], nt = F.tileM_start[l + 1] - t0;
+      if (nt <= 0) continue;
+      hipLaunchKernelGGL((m2l_kernel<KP, 0>), dim3(8 * ((nt + 7) / 8)), dim3(2 * KP), 0,
+                         ctx->stream, F.Mt.get(), F.Mt.as<float>(), tab.W_m2m.get(),
+                         F.tilesM.as<M2LTile>() + t0, nt, F.box_ids.as<int32_t>(),
+                         F.child_tab.as<int32_t>(), 8, tab.slot_tab.as<int32_t>(), tab.slot_ld,
+                         F.tlistM.as<int32_t>() + static_cast<int64_t>(t0) * tab.slot_ld,
+                         F.tnM.as<int32_t>() + t0, tab.slot_ld, 0);
       MF_LAUNCH_CHECK();
     }
   }
@@ -1927,12 +1970,18 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
     }
   }
   {
+    // L2L on the M2L kernel (FP32 MFMA, accumulating): the children of level
+    // l (tiles of one octant class) add their parent's local expansion.
     KernelTimer timer(ctx, "l2l");
-    for (int l = 2; l < D; ++l) {
-      hipLaunchKernelGGL((l2l_kernel<KP>), dim3(F.count[l + 1]), dim3(64), 0, ctx->stream,
-                         F.count[l + 1], F.base[l + 1], F.box_parent.as<int32_t>() + F.base[l + 1],
-                         F.box_keys.as<uint64_t>() + F.base[l + 1], tab.W_l2l.as<float>(), ND,
-                         F.Lt.as<float>());
+    for (int l = 3; l <= D; ++l) {
+      const int t0 = F.tileL_start[l], nt = F.tileL_start[l + 1] - t0;
+      if (nt <= 0) continue;
+      hipLaunchKernelGGL((m2l_kernel<KP, 0>), dim3(8 * ((nt + 7) / 8)), dim3(2 * KP), 0,
+                         ctx->stream, F.Lt.get(), F.Lt.as<float>(), tab.W_l2l.get(),
+                         F.tilesL.as<M2LTile>() + t0, nt, F.class_ids2.as<int32_t>(),
+                         F.box_parent.as<int32_t>(), 1, tab.slot_tab.as<int32_t>(), tab.slot_ld,
+                         F.tlistL.as<int32_t>() + static_cast<int64_t>(t0) * tab.slot_ld,
+                         F.tnL.as<int32_t>() + t0, tab.slot_ld, 1);
       MF_LAUNCH_CHECK();
     }
   }
