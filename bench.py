@@ -434,11 +434,19 @@ def main():
     else:
         roofline = kernel_roofline("field", float(n) * n * FIELD_FLOPS_PER_PAIR, FP32_PEAK_TFLOPS,
                                    "valu")
-    if args.traffic and os.path.exists(args.traffic):
-        with open(args.traffic) as f:
+    traffic_path = args.traffic or os.path.join(
+        ROOT, "profiles", f"traffic_{mode_name}_{system.name}.json")
+    if os.path.exists(traffic_path):
+        # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+        # summarised under profiles/ (one file per workload, one entry per kernel)
+        with open(traffic_path) as f:
             t = json.load(f)
-        if t.get("workload") == system.name and t.get("kernel") == roofline["kernel"]:
-            roofline["traffic"] = t["hbm_bytes_per_launch"]
+        if t.get("workload") == system.name:
+            ks = t.get("kernels", {})
+            for r in [roofline] + roofline.get("others", []):
+                if r["kernel"] in ks:
+                    r["traffic"] = ks[r["kernel"]]["hbm_bytes_per_launch"]
+                    r["traffic_source"] = os.path.relpath(traffic_path, ROOT)
 
     compare = None
     if mode == magforce.MODE_FMM and args.m2l != "fp32" and not args.no_compare:
