@@ -135,6 +135,27 @@ def measure_allpairs(ctx, log_prefix="allpairs"):
            "kernel_ms": avg_ms, "wall_ms_per_matvec": wall / reps * 1e3,
            "tflops": pairs * FIELD_FLOPS_PER_PAIR / (avg_ms * 1e-3) / 1e12}
     res["frac_fp32_peak"] = res["tflops"] / FP32_PEAK_TFLOPS
+    # the cylinder's own step: GMRES(30) run for 30 all-pairs products, then the
+    # force; scored against the FP64 solution (all-pairs FP64 refinement to
+    # 1e-12) and FP64 forces on 4096 seeded samples
+    b = magforce.uniform_field(s.n, s.hconst)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    sol = plan.solve(s.kappa, b, method=magforce.SOLVE_GMRES, restart=30, maxit=30, rtol=1e-30)
+    F = plan.force(s.kappa * sol.x)
+    e1.record()
+    torch.cuda.synchronize()
+    step = {"ms": e0.elapsed_time(e1), "solver": sol.stats}
+    ref = plan.solve_f64(s.kappa, b.double(), rtol=1e-12, max_refine=6)
+    rng = np.random.default_rng(12345)
+    idx = torch.tensor(np.sort(rng.choice(s.n, size=4096, replace=False)), dtype=torch.int64,
+                       device="cuda")
+    F_ref = magforce.force_subset_f64(ctx, pos.double(), s.kappa * ref.x, s.h, idx)
+    step["force_rel_l2"] = float(torch.linalg.norm(F[:, idx].double() - F_ref) /
+                                 torch.linalg.norm(F_ref))
+    step["x_rel_l2"] = float(torch.linalg.norm(sol.x.double() - ref.x) / torch.linalg.norm(ref.x))
+    res["gmres30_step"] = step
     log(f"{log_prefix}: {res}")
     plan.close()
     return res
@@ -301,6 +322,54 @@ def run_droplets(ctx, args, m2l_code):
     return out
 
 
+def run_c1(ctx):
+    """The reference's CPU-sized case (8,192 jittered lattice, h = 2, chi = 1):
+    GPU all-pairs solve (CG to 1e-6) + force, next to the CPU oracle doing the
+    same in FP64 (CG to 1e-6) and the FP32 restatement of the reference's
+    kernels timed for one field and one force product."""
+    import torch
+    import magforce
+    from magforce import configs
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    s = configs.lattice8k()
+    threads = min(16, os.cpu_count() or 1)
+    pos = to_dev(s.pos)
+    b = magforce.uniform_field(s.n, s.hconst)
+    plan = magforce.Plan(ctx, pos, s.h)
+    plan.solve(s.kappa, b, rtol=1e-6)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    sol = plan.solve(s.kappa, b, rtol=1e-6)
+    F = plan.force(s.kappa * sol.x)
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1)
+    p64 = s.pos.astype(np.float64)
+    bh = np.tile(np.asarray(s.hconst, np.float64), (s.n, 1))
+    t0 = time.perf_counter()
+    x64, it, _ = oracle.cg_solve(p64, s.h, s.kappa, bh, rtol=1e-6, nthreads=threads)
+    F64 = oracle.force_allpairs(p64, s.kappa * x64, s.h, nthreads=threads)
+    cpu64 = time.perf_counter() - t0
+    m32 = (s.kappa * x64).astype(np.float32)
+    t0 = time.perf_counter()
+    oracle.field_allpairs_f32ref(s.pos, m32, s.h, nthreads=threads)
+    t_f32_field = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.force_allpairs_f32ref(s.pos, m32, s.h, nthreads=threads)
+    t_f32_force = time.perf_counter() - t0
+    err = float(np.linalg.norm(F.double().cpu().numpy().T - F64) / np.linalg.norm(F64))
+    out = {"workload": "lattice8k", "n": s.n, "gpu_allpairs_ms": gpu_ms,
+           "gpu_solver": sol.stats, "gpu_force_rel_l2_vs_cpu_fp64": err,
+           "cpu_fp64_solve_force_s": cpu64, "cpu_fp64_cg_iterations": it,
+           "cpu_f32_restatement_field_s": t_f32_field,
+           "cpu_f32_restatement_force_s": t_f32_force, "cpu_threads": threads}
+    plan.close()
+    log(f"c1: {out}")
+    return out
+
+
 def cpu_baseline(system, budget_s, matvecs_per_step):
     """The CPU oracle (FP64 all-pairs restatement) on a bounded sample of
     target rows, extrapolated to one solve + force step of the workload."""
@@ -453,6 +522,7 @@ def main():
         compare = time_fp32_fmm(ctx, system, args)
         compare["speedup"] = compare["ms_per_step"] / ms_step
     batch = None if args.no_batch else run_droplets(ctx, args, m2l_code)
+    c1 = None if args.no_cpu else run_c1(ctx)
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)
     if allpairs is not None:
         # plain FP32 all-pairs step at this n, from the measured pair rates
@@ -482,6 +552,7 @@ def main():
         "error": check,
         "fp32_fmm": compare,
         "batch": batch,
+        "c1_lattice8k": c1,
         "allpairs": allpairs,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -888,6 +888,15 @@ __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec
   }
 }
 
+// Workgroup -> list position with each XCD (round-robin dispatch over 8)
+// walking a contiguous range: neighbouring leaves share most of their near
+// sources, which then hit in that XCD's L2.  Returns -1 for the padding.
+__device__ __forceinline__ int xcd_contiguous(int n) {
+  const int per_xcd = (n + 7) / 8;
+  const int pos = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+  return pos < n ? pos : -1;
+}
+
 // Targets [tb, te) of a leaf of cnt particles handled by the T-shaped kernel.
 template <int T>
 __device__ __forceinline__ void p2p_target_span(int cnt, int &tb, int &te) {
@@ -924,13 +933,15 @@ __device__ __forceinline__ double group_sum(double v) {
 // leaves: the leaf indices this launch covers (blockIdx.x -> leaf).
 template <int T>
 __global__ __launch_bounds__(64) void p2p_field_kernel(
-    const int32_t *__restrict__ leaves, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
     int nslot, int accumulate, FieldConsts fc, float hc0, float hc1, float hc2,
     float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
   constexpr int G = 64 / T;
-  const int b = leaves[blockIdx.x];
+  const int li = xcd_contiguous(nleaves);
+  if (li < 0) return;
+  const int b = leaves[li];
   const int lane = threadIdx.x;
   const int tl = T == 64 ? lane : lane % T;
   const int g = T == 64 ? 0 : lane / T;
@@ -976,22 +987,24 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
 // Leaf shapes for the near-field kernels.  List 0 (T = 64) holds every leaf
 // with a full chunk of 64 or a remainder of more than 32; lists 1..3 hold the
 // leaves whose remainder (cnt mod 64) is 17..32, 9..16 or 1..8.  A leaf can
-// be in list 0 and in one of the others: they cover disjoint targets.
+// be in list 0 and in one of the others: they cover disjoint targets.  Two
+// (list, leaf) entries per leaf (list 4 = none); a stable sort by list then
+// gives every list in leaf (Morton) order.
 __global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ count,
-                                       int32_t *__restrict__ lists, int32_t *__restrict__ sizes) {
+                                       uint64_t *__restrict__ key, int32_t *__restrict__ leaf,
+                                       int32_t *__restrict__ sizes) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nleaf) return;
   const int c = count[b];
   const int r = c & 63;
-  if (c >= 64 || r > 32) {
-    const int at = atomicAdd(&sizes[0], 1);
-    lists[at] = b;
-  }
-  if (r > 0 && r <= 32) {
-    const int k = r > 16 ? 1 : (r > 8 ? 2 : 3);
-    const int at = atomicAdd(&sizes[k], 1);
-    lists[static_cast<int64_t>(k) * nleaf + at] = b;
-  }
+  const int k0 = (c >= 64 || r > 32) ? 0 : 4;
+  const int k1 = (r > 0 && r <= 32) ? (r > 16 ? 1 : (r > 8 ? 2 : 3)) : 4;
+  key[2 * b] = static_cast<uint64_t>(k0);
+  key[2 * b + 1] = static_cast<uint64_t>(k1);
+  leaf[2 * b] = b;
+  leaf[2 * b + 1] = b;
+  if (k0 < 4) atomicAdd(&sizes[0], 1);
+  if (k1 < 4) atomicAdd(&sizes[k1], 1);
 }
 
 struct Text9 {
@@ -1052,12 +1065,14 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
 // Leaf shapes (T) as in p2p_field_kernel.
 template <int T>
 __global__ __launch_bounds__(64) void p2p_force_kernel(
-    const int32_t *__restrict__ leaves, const int32_t *__restrict__ start,
+    const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
     int nslot, HessConsts hc, float *__restrict__ fx, float *__restrict__ fy,
     float *__restrict__ fz) {
   constexpr int G = 64 / T;
+  const int li = xcd_contiguous(nleaves);
+  if (li < 0) return;  // uniform over the workgroup, before the barrier
   __shared__ float4 s_tab[12];
   if (threadIdx.x < 12) {
     const float *row = &hc.curve[threadIdx.x / 3][(threadIdx.x % 3) * 4];
@@ -1065,7 +1080,7 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   }
   __syncthreads();
   const float *tab = reinterpret_cast<const float *>(s_tab);
-  const int b = leaves[blockIdx.x];
+  const int b = leaves[li];
   const int lane = threadIdx.x;
   const int tl = T == 64 ? lane : lane % T;
   const int g = T == 64 ? 0 : lane / T;
@@ -1269,8 +1284,9 @@ struct FmmState {
   DeviceBuffer nearF, nearH; // leaf lists
   DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
   DeviceBuffer nrF, nrH;     // int2 range counts per leaf (class 0, class 1)
-  DeviceBuffer leaf_lists;   // int32 [4][nleaf]: leaves of > 32, <= 32, <= 16, <= 8 particles
+  DeviceBuffer leaf_lists;   // int32: the four leaf-shape lists back to back, each in leaf order
   int leaf_list_n[4] = {0, 0, 0, 0};
+  int leaf_list_off[4] = {0, 0, 0, 0};
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
@@ -1628,17 +1644,24 @@ Status fmm_build(mf_plan *plan) {
                      F.box_count.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>());
   MF_LAUNCH_CHECK();
   {
-    // leaf shapes of the near-field kernels
-    MF_TRY(need<int32_t>(F.leaf_lists, 4 * static_cast<int64_t>(nleaf)));
-    void *pz = nullptr;
+    // leaf shapes of the near-field kernels, each list in leaf order
+    const int64_t ne = 2 * static_cast<int64_t>(nleaf);
+    MF_TRY(need<int32_t>(F.leaf_lists, ne));
+    void *pk = nullptr, *pz = nullptr;
+    MF_TRY(workspace(ctx, "fmm_leaf_keys", (sizeof(uint64_t) * 2 + sizeof(int32_t)) * ne, &pk));
     MF_TRY(workspace(ctx, "fmm_leaf_sizes", sizeof(int32_t) * 4, &pz));
+    uint64_t *k_in = static_cast<uint64_t *>(pk), *k_out = k_in + ne;
+    int32_t *v_in = reinterpret_cast<int32_t *>(k_out + ne);
     MF_HIP(hipMemsetAsync(pz, 0, sizeof(int32_t) * 4, ctx->stream));
     hipLaunchKernelGGL(classify_leaves_kernel, dim3(grid_for(nleaf, 256)), dim3(256), 0,
-                       ctx->stream, nleaf, F.box_count.as<int32_t>(), F.leaf_lists.as<int32_t>(),
+                       ctx->stream, nleaf, F.box_count.as<int32_t>(), k_in, v_in,
                        static_cast<int32_t *>(pz));
     MF_LAUNCH_CHECK();
+    MF_TRY(sort_pairs_u64_i32(ctx, k_in, k_out, v_in, F.leaf_lists.as<int32_t>(), ne, 0, 3));
     MF_HIP(hipMemcpyAsync(F.leaf_list_n, pz, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, ctx->stream));
     MF_HIP(hipStreamSynchronize(ctx->stream));
+    F.leaf_list_off[0] = 0;
+    for (int k = 1; k < 4; ++k) F.leaf_list_off[k] = F.leaf_list_off[k - 1] + F.leaf_list_n[k - 1];
   }
   // 5. M2L source slots (levels >= 2) and the class-sorted tile lists
   MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox) * tab.nslotF));
@@ -2023,8 +2046,9 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FIELD(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(F.leaf_list_n[k]), dim3(64), 0, ctx->stream, \
-                       lists + static_cast<int64_t>(k) * nleaf, F.box_first.as<int32_t>(),       \
+    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+                       0, ctx->stream, lists + F.leaf_list_off[k],                             \
+                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
                        F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, use_far, plan->fc, h0, h1,  \
                        h2, ox, oy, oz);                                                          \
@@ -2073,8 +2097,9 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
   const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FORCE(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(F.leaf_list_n[k]), dim3(64), 0, ctx->stream, \
-                       lists + static_cast<int64_t>(k) * nleaf, F.box_first.as<int32_t>(),       \
+    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+                       0, ctx->stream, lists + F.leaf_list_off[k],                             \
+                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
                        F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, fx, fy, fz);     \
     MF_LAUNCH_CHECK();                                                                           \
