@@ -250,9 +250,11 @@ def check_error_subsample(ctx, st: Stepper, samples: int):
 def run_droplets(ctx, args, m2l_code):
     """The 96 uneven droplets (2,048..262,144 particles each, chi in [0.5, 3],
     random unit H directions) as one batch plan: one step = batch plan
-    (re)build + batched CG to rtol per system + block-diagonal force.  Error:
-    per droplet, up to 128 seeded particles, FP64 force at the GPU moments
-    (that droplet's sources) and the FP64 residual b - x + kappa G x."""
+    (re)build + batched CG to rtol per system (one CG iteration captured in a
+    hipGraph and replayed) + block-diagonal force.  The expansion order comes
+    from the autotuner run on the largest droplet.  Error: per droplet, up to
+    128 seeded particles, FP64 force at the GPU moments (that droplet's
+    sources) and the FP64 residual b - x + kappa G x."""
     import torch
     import magforce
     from magforce import configs
@@ -267,13 +269,24 @@ def run_droplets(ctx, args, m2l_code):
     kpart = torch.tensor(np.concatenate([np.full(sy.n, sy.kappa, np.float32) for sy in systems]),
                          device="cuda")
     state = {"plan": None, "x": None, "F": torch.empty_like(b), "stats": None}
+    # order and M2L precision from the autotuner, run on the largest droplet
+    # (target 1e-5 relative field error against FP64); --m2l fixes the precision
+    from magforce import tune
+    big = max(systems, key=lambda sy: sy.n)
+    best, cands = tune.autotune_fmm(ctx, to_dev(big.pos), big.h, target_rel_err=1e-5,
+                                    precisions=(m2l_code,))
+    order = best.order if best else 10
+    tuned = {"on": big.name, "n": big.n, "chosen": best.as_dict() if best else None,
+             "candidates": [c.as_dict() for c in cands]}
+    log(f"droplets96 autotune: {tuned}")
 
     def step():
         if state["plan"] is None:
-            state["plan"] = magforce.BatchPlan(ctx, pos, off, systems[0].h, m2l_precision=m2l_code)
+            state["plan"] = magforce.BatchPlan(ctx, pos, off, systems[0].h, order=order,
+                                               m2l_precision=m2l_code)
         else:
             state["plan"].update_positions(pos)
-        res = state["plan"].solve(kappas, b, rtol=args.rtol)
+        res = state["plan"].solve(kappas, b, rtol=args.rtol, graph_iters=2)
         state["x"], state["stats"] = res.x, res.stats
         state["plan"].force(res.x * kpart, out=state["F"])
 
@@ -290,7 +303,8 @@ def run_droplets(ctx, args, m2l_code):
     st = state["stats"]
     out = {"workload": "droplets96", "systems": len(systems), "n": off[-1],
            "n_min": min(sy.n for sy in systems), "n_max": max(sy.n for sy in systems),
-           "ms_per_step": ms, "m2l_precision": args.m2l,
+           "ms_per_step": ms, "m2l_precision": args.m2l, "order": order, "autotune": tuned,
+           "graph_iters": 2,
            "solver": {"iterations": st["iterations"], "matvecs": st["matvecs"],
                       "converged": st["converged"],
                       "max_sys_rel_residual": max(st["sys_rel_residual"])},

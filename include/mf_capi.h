@@ -205,7 +205,9 @@ typedef struct mf_solve_opts {
   int true_residual; /* 1: recompute ||b - A x|| with a fresh product at exit */
   int zero_start;    /* 1: start from x = 0 (x is overwritten); the first residual
                         is b itself, which saves one product */
-  int reserved[1];
+  int graph_iters;   /* mf_plan_solve_batch: > 0 captures one CG iteration (after one
+                        eager iteration) in a hipGraph and replays it graph_iters
+                        times between convergence checks; 0 = plain launches */
 } mf_solve_opts;
 
 typedef struct mf_solve_stats {

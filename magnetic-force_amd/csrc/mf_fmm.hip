@@ -495,10 +495,9 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
   using GA = M2LGather<KP, NPART>;
   int src[GA::IT];
   float4 v[GA::IT];
-  int any = 0;
   if (na > 0) {
     m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[0], tid, src);
-    any = m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
+    (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
     if (na > 1) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[1], tid, src);
   }
   for (int pc = 0; pc < na; ++pc) {
@@ -516,13 +515,15 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
         *reinterpret_cast<float4 *>(&sB[(p * kTileT + j) * LDB + 8 * q8]) = v[it];
       }
     }
-    const int go = __syncthreads_or(any);
+    // a plain barrier: the tile lists keep only offsets with a source, so no
+    // block-wide "any source" vote (an LDS atomic per thread) is needed
+    __syncthreads();
     const int tix = smap[tl[pc]];
     if (pc + 1 < na) {
-      any = m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
+      (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
       if (pc + 2 < na) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[pc + 2], tid, src);
     }
-    if (go) {
+    {
       if constexpr (NPART == 0) {
         const float *sF = reinterpret_cast<const float *>(s_raw);
         const float *__restrict__ A = static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP;
@@ -1003,8 +1004,13 @@ __global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ co
   key[2 * b + 1] = static_cast<uint64_t>(k1);
   leaf[2 * b] = b;
   leaf[2 * b + 1] = b;
-  if (k0 < 4) atomicAdd(&sizes[0], 1);
-  if (k1 < 4) atomicAdd(&sizes[k1], 1);
+  // one atomic per wave and list instead of one per leaf
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint64_t m = __ballot((k == 0 && k0 == 0) || k1 == k);
+    if (m != 0 && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1)
+      atomicAdd(&sizes[k], __popcll(m));
+  }
 }
 
 struct Text9 {

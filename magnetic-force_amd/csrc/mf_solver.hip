@@ -910,10 +910,9 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   MF_LAUNCH_CHECK();
   double mx = 0.0;
   MF_TRY(read_scalar(ctx, S.maxratio, &mx));
-  int it = 0;
-  while (mx > rtol && st->matvecs < maxit) {
+  // one CG iteration, enqueued on ctx->stream without host reads
+  auto body = [&]() -> Status {
     MF_TRY(plan_apply_G(plan, 1.f, w.p, w.p + n, w.p + 2 * n, nullptr, w.g, w.g + n, w.g + 2 * n));
-    st->matvecs += 1;
     hipLaunchKernelGGL(bcg_ap_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, p, g,
                        v3(w.ap, n));
     MF_LAUNCH_CHECK();
@@ -929,7 +928,62 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
     hipLaunchKernelGGL(bcg_p_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, sys_of,
                        static_cast<const double *>(S.beta), r, v3(w.p, n));
     MF_LAUNCH_CHECK();
-    ++it;
+    return Status::Ok();
+  };
+  // Graph mode: after one eager iteration (every workspace exists by then) one
+  // iteration is captured and replayed graph_iters times per convergence
+  // check.  The capture and the replays run on a non-null stream (the
+  // context's own when it works on the null stream), ordered with the
+  // context's stream by events.  Converged systems are frozen, so the extra
+  // replays past convergence only cost time.
+  struct GraphState {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~GraphState() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      if (graph) (void)hipGraphDestroy(graph);
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } gs;
+  const int graph_iters = opts.graph_iters;
+  hipStream_t work = ctx->stream ? ctx->stream : ctx->own_stream;
+  int it = 0;
+  while (mx > rtol && st->matvecs < maxit) {
+    if (graph_iters > 0 && it >= 1) {
+      if (!gs.exec) {
+        MF_HIP(hipEventCreateWithFlags(&gs.ev[0], hipEventDisableTiming));
+        MF_HIP(hipEventCreateWithFlags(&gs.ev[1], hipEventDisableTiming));
+        hipStream_t saved = ctx->stream;
+        const bool timing = ctx->timing;
+        MF_HIP(hipEventRecord(gs.ev[0], saved));
+        MF_HIP(hipStreamWaitEvent(work, gs.ev[0], 0));
+        ctx->stream = work;
+        ctx->timing = false;  // no timing events inside the graph
+        hipError_t e0 = hipStreamBeginCapture(work, hipStreamCaptureModeThreadLocal);
+        Status sb = e0 == hipSuccess ? body() : Status::Ok();
+        hipError_t e1 = e0 == hipSuccess ? hipStreamEndCapture(work, &gs.graph) : e0;
+        ctx->stream = saved;
+        ctx->timing = timing;
+        MF_TRY(sb);
+        MF_HIP(e1);
+        MF_HIP(hipGraphInstantiate(&gs.exec, gs.graph, nullptr, nullptr, 0));
+      } else {
+        MF_HIP(hipEventRecord(gs.ev[0], ctx->stream));
+        MF_HIP(hipStreamWaitEvent(work, gs.ev[0], 0));
+      }
+      const int k = std::min(graph_iters, maxit - st->matvecs);
+      for (int j = 0; j < k; ++j) MF_HIP(hipGraphLaunch(gs.exec, work));
+      MF_HIP(hipEventRecord(gs.ev[1], work));
+      MF_HIP(hipStreamWaitEvent(ctx->stream, gs.ev[1], 0));
+      st->matvecs += k;
+      it += k;
+    } else {
+      MF_TRY(body());
+      st->matvecs += 1;
+      ++it;
+    }
     MF_TRY(read_scalar(ctx, S.maxratio, &mx));
     if (!std::isfinite(mx)) return Status::Err(MF_ERR_NOT_CONVERGED, "CG residual is not finite");
   }

@@ -72,7 +72,7 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("maxit", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("refine_fp64", ctypes.c_int),
                 ("true_residual", ctypes.c_int), ("zero_start", ctypes.c_int),
-                ("reserved", ctypes.c_int * 1)]
+                ("graph_iters", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -465,14 +465,15 @@ class BatchPlan(Plan):
         self.h_plan = hp
         self.pos = p
 
-    def solve(self, kappas, b, x0=None, rtol: float = 1e-6, maxit: int = 1000):
+    def solve(self, kappas, b, x0=None, rtol: float = 1e-6, maxit: int = 1000,
+              graph_iters: int = 0):
         import torch
         bb = _soa(b, self.n)
         if len(kappas) != self.nsys:
             raise ValueError("one kappa per system")
         x = torch.empty_like(bb) if x0 is None else _soa(x0, self.n).clone()
         opts = SolveOpts(method=SOLVE_CG, maxit=maxit, rtol=rtol,
-                         zero_start=1 if x0 is None else 0)
+                         zero_start=1 if x0 is None else 0, graph_iters=graph_iters)
         st = SolveStats()
         kap = (ctypes.c_double * self.nsys)(*[float(k) for k in kappas])
         res = (ctypes.c_double * self.nsys)()

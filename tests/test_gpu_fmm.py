@@ -157,3 +157,19 @@ def test_fmm_matches_fp64_gpu_at_100k(ctx):
     # random moments: the field is small where it cancels, so the relative
     # error runs above that of a solved (smooth) magnetisation
     assert rel_err(got, want) < 5e-6
+
+
+def test_autotuner_picks_fastest_within_target(ctx, slab_case):
+    """The autotuner's choice meets the error target on an independent check,
+    and no faster candidate also met it."""
+    import magforce
+    from magforce import tune
+    pos, m, field, _ = slab_case
+    best, cands = tune.autotune_fmm(ctx, dev(pos), 2.0, target_rel_err=1e-5, samples=512)
+    assert best is not None and best.rel_err <= 1e-5
+    assert len(cands) == 9
+    faster_ok = [c for c in cands if c.rel_err <= 1e-5 and c.ms_per_product < best.ms_per_product]
+    assert not faster_ok
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=best.order,
+                         m2l_precision=best.m2l_precision)
+    assert rel_err(host(plan.field(dev(m))), field) < 1e-5
