@@ -919,6 +919,67 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// Near-source stream of a sparse leaf (the T < 64 shapes).  The leaf's near
+// ranges, concatenated, are walked in windows of 64 records: lane l loads
+// record w0 + l (one coalesced load per window, issued while the previous
+// window is evaluated), the window goes through LDS, and lane group g takes
+// records g, g + G, ... of it.  A range-by-range walk instead waits once per
+// G sources on a load, which dominated these leaves' time.
+struct NearStream {
+  int pre[kNearH + 1];  // stream offset of each range; pre[nrange] = total
+  int beg[kNearH];      // first record of each range
+  float4 win[2][64];    // the window: record halves {x, y, z, -}, {mx, my, mz, -}
+};
+
+// Fills pre/beg from the leaf's ranges (one wave); returns the total.
+__device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__restrict__ rg,
+                                                 int nrange, int lane) {
+  int carry = 0;
+  for (int c = 0; c < nrange; c += 64) {
+    const int r = c + lane;
+    int len = 0;
+    if (r < nrange) {
+      const int2 q = rg[r];
+      len = q.y - q.x;
+      S.beg[r] = q.x;
+    }
+    int incl = len;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += v;
+    }
+    if (r < nrange) S.pre[r] = carry + incl - len;
+    carry += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) S.pre[nrange] = carry;
+  __syncthreads();
+  return carry;
+}
+
+// Record of stream position i < total: the last range r with pre[r] <= i
+// (empty ranges share their successor's offset and are skipped by "last").
+__device__ __forceinline__ int near_stream_record(const NearStream &S, int nrange, int i) {
+  int lo = 0, hi = nrange - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.pre[mid] <= i) lo = mid; else hi = mid - 1;
+  }
+  return S.beg[lo] + (i - S.pre[lo]);
+}
+
+// Loads the window starting at w0 into (a, b) (lane l: position w0 + l).
+__device__ __forceinline__ void near_stream_fetch(const NearStream &S, const SrcRec *__restrict__ rec,
+                                                  int nrange, int total, int w0, int lane,
+                                                  float4 &a, float4 &b) {
+  const int i = w0 + lane;
+  if (i < total) {
+    const float4 *p = reinterpret_cast<const float4 *>(rec + near_stream_record(S, nrange, i));
+    a = p[0];
+    b = p[1];
+  }
+}
+
 // P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
 // writes H_const + near (trees without a far field).  One wave per leaf, in
 // one of two shapes:
@@ -952,6 +1013,63 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
   const int2 nrb = nr[b];
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double cf = 1.0 / (4.0 * kPi);
+  if constexpr (T < 64) {
+    // at most T targets: one pass over the near-source stream
+    __shared__ NearStream S;
+    const int nrange = nrb.x + nrb.y;
+    const int total = near_stream_setup(S, rg, nrange, lane);
+    if (tb >= te) return;  // uniform, after the setup barrier
+    const int t = tb + tl;
+    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
+    const float tx = me.x, ty = me.y, tz = me.z;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
+    near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
+    for (int w0 = 0; w0 < total; w0 += 64) {
+      __syncthreads();  // the previous window is consumed
+      S.win[0][lane] = wa;
+      S.win[1][lane] = wb;
+      __syncthreads();
+      if (w0 + 64 < total) near_stream_fetch(S, rec, nrange, total, w0 + 64, lane, wa, wb);
+      const int n = min(64, total - w0);
+      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
+      for (int k = g; k < n; k += G) {
+        const float4 pa = S.win[0][k], pb = S.win[1][k];
+        const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
+        const float mx = pb.x, my = pb.y, mz = pb.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float ir2 = ir * ir;
+        float beta = ir2 * ir;
+        float alpha = beta * ir2;
+        if (r2 < fc.near2) field_near_coeffs(fc, r2, ir, alpha, beta);
+        const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+        const float s1 = dm * alpha;
+        a1x = fmaf(dx, s1, a1x);
+        a1y = fmaf(dy, s1, a1y);
+        a1z = fmaf(dz, s1, a1z);
+        a2x = fmaf(mx, beta, a2x);
+        a2y = fmaf(my, beta, a2y);
+        a2z = fmaf(mz, beta, a2z);
+      }
+      acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
+      acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
+      acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
+    }
+    acc0 = group_sum<T>(acc0);
+    acc1 = group_sum<T>(acc1);
+    acc2 = group_sum<T>(acc2);
+    if (g == 0 && t < te) {
+      const int o = perm[s0 + t];
+      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
+      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
+      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
+      ox[o] = static_cast<float>(b0 + cf * acc0);
+      oy[o] = static_cast<float>(b1 + cf * acc1);
+      oz[o] = static_cast<float>(b2 + cf * acc2);
+    }
+    return;
+  }
   for (int t0 = tb; t0 < te; t0 += T) {
     const int t = t0 + tl;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
@@ -1096,6 +1214,70 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   const int nrange = nr[b].x;
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double sfac = 3.0 * kMu0 / (4.0 * kPi);
+  if constexpr (T < 64) {
+    // at most T targets: one pass over the near-source stream
+    __shared__ NearStream S;
+    const int total = near_stream_setup(S, rg, nrange, lane);
+    if (tb >= te) return;  // uniform, after the setup barrier
+    const int t = tb + tl;
+    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
+    const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
+    double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
+    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
+    near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
+    for (int w0 = 0; w0 < total; w0 += 64) {
+      __syncthreads();  // the previous window is consumed
+      S.win[0][lane] = wa;
+      S.win[1][lane] = wb;
+      __syncthreads();
+      if (w0 + 64 < total) near_stream_fetch(S, rec, nrange, total, w0 + 64, lane, wa, wb);
+      const int n = min(64, total - w0);
+      float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
+      for (int k = g; k < n; k += G) {
+        const float4 pa = S.win[0][k], pb = S.win[1][k];
+        const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
+        const float mx = pb.x, my = pb.y, mz = pb.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float ir = __builtin_amdgcn_rsqf(r2);
+        const float ir2 = ir * ir;
+        float Pc = ir2 * ir2 * ir;
+        float Rc = -5.f * Pc * ir2;
+        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.curve_scale, hc.inv_h, r2, ir, Pc, Rc);
+        const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+        const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+        const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
+        const float e = fmaf(Rc, dm * dn, Pc * mn);
+        const float f = Pc * dn;
+        dx0 = fmaf(dx, e, dx0);
+        dy0 = fmaf(dy, e, dy0);
+        dz0 = fmaf(dz, e, dz0);
+        mx0 = fmaf(mx, f, mx0);
+        my0 = fmaf(my, f, my0);
+        mz0 = fmaf(mz, f, mz0);
+        n0 = fmaf(Pc, dm, n0);
+      }
+      ad0 += dx0;
+      ad1 += dy0;
+      ad2 += dz0;
+      am0 += mx0;
+      am1 += my0;
+      am2 += mz0;
+      an += n0;
+    }
+    double f0 = ad0 + am0 + static_cast<double>(nx) * an;
+    double f1 = ad1 + am1 + static_cast<double>(ny) * an;
+    double f2 = ad2 + am2 + static_cast<double>(nz) * an;
+    f0 = group_sum<T>(f0);
+    f1 = group_sum<T>(f1);
+    f2 = group_sum<T>(f2);
+    if (g == 0 && t < te) {
+      const int o = perm[s0 + t];
+      fx[o] = static_cast<float>(static_cast<double>(fx[o]) + sfac * f0);
+      fy[o] = static_cast<float>(static_cast<double>(fy[o]) + sfac * f1);
+      fz[o] = static_cast<float>(static_cast<double>(fz[o]) + sfac * f2);
+    }
+    return;
+  }
   for (int t0 = tb; t0 < te; t0 += T) {
     const int t = t0 + tl;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
