@@ -367,9 +367,10 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
 //    three with i + j <= 1 (~2^-16).  Accumulation is FP32 in the MFMA.  A
 //    BF16 MFMA does 16x the multiply-adds of the FP32 one per cycle, so six
 //    products cost 3/8 of the FP32 MFMA time.  The offset matrices are stored
-//    pre-split (from their FP64 values), row-major, so that a lane's eight
-//    consecutive k of one row are one 16-byte load; the gathered multipoles
-//    are split as they are staged into LDS.
+//    pre-split (from their FP64 values) in fragment order: a lane's eight
+//    consecutive k of one row are one 16-byte load, and a wave's fragment is
+//    1 KB of consecutive memory (whole cache lines).  The multipoles are
+//    pre-split once per product (split_multipoles_kernel).
 constexpr int kTileT = 64;
 constexpr int kRowM2M = 16, kRowL2L = 17, kSlotRows = 25;  // slot-table rows
 
@@ -553,7 +554,8 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
 #pragma unroll
             for (int p = 0; p < NPART; ++p)
               a[rb][p] = *reinterpret_cast<const bf16x8 *>(
-                  A + (static_cast<int64_t>(p) * KP + 32 * wave + 16 * rb + r16) * KP + kk);
+                  A + ((static_cast<int64_t>(p * (KP / 32) + k0 / 32) * (KP / 16) + 2 * wave + rb) * 64 +
+                       lane) * 8);
 #pragma unroll
           for (int cb = 0; cb < 4; ++cb) {
             bf16x8 b[NPART];
@@ -1441,7 +1443,7 @@ struct FmmTables {
   int P = 0, KP = 0;
   int nslotF = 0, nslotH = 0;
   DeviceBuffer T;         // [noff][KP][KP] column-major
-  DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP rows][KP] row-major
+  DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP x KP in fragment order]
   DeviceBuffer W_m2m;     // [8][KP][KP]
   DeviceBuffer W_l2l;     // [8][KP][KP]
   DeviceBuffer slot_tab;  // [kSlotRows][slot_ld] table index per slot (-1 unused)
@@ -1507,6 +1509,18 @@ uint16_t bf16_bits_rne_host(float x) {
   return static_cast<uint16_t>((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
 
+// Position of element (row, col) of BF16 part p within one offset's table.
+// The order is the MFMA fragment order of the M2L kernel: per (part, 32-wide
+// k block, 16-row block), the 64 lanes' 8-element row pieces (lane = 16 kq +
+// r16 holds row 16 rb + r16, columns 32 blk + 8 kq .. + 7) lie in lane order,
+// so that one wave's fragment load reads 1 KB of consecutive memory.
+size_t bf16_frag_index(int KP, int p, int row, int col) {
+  const int blk = col / 32, kq = (col % 32) / 8, e = col % 8;
+  const int rb = row / 16, r16 = row % 16;
+  const size_t piece = ((static_cast<size_t>(p) * (KP / 32) + blk) * (KP / 16) + rb) * 64 + 16 * kq + r16;
+  return piece * 8 + e;
+}
+
 int kp_for(int P) {
   const int nd = (P + 1) * (P + 1);
   return ((nd + 31) / 32) * 32;
@@ -1565,7 +1579,7 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
       hoh[b * t->nslotH + k] = oh[b][k];
     }
   }
-  // M2L tables: FP32 column-major, and BF16 parts row-major
+  // M2L tables: FP32 column-major, and BF16 parts in fragment order
   std::vector<float> T(static_cast<size_t>(uniq.size()) * KP * KP);
   std::vector<uint16_t> Tb(static_cast<size_t>(uniq.size()) * 3 * KP * KP);
   for (size_t u = 0; u < uniq.size(); ++u) {
@@ -1576,7 +1590,7 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
         double r = m[static_cast<size_t>(col) * KP + row];
         for (int p = 0; p < 3; ++p) {
           const uint16_t bits = bf16_bits_rne_host(static_cast<float>(r));
-          Tb[((u * 3 + p) * KP + row) * KP + col] = bits;
+          Tb[u * 3 * KP * KP + bf16_frag_index(KP, p, row, col)] = bits;
           uint32_t w = static_cast<uint32_t>(bits) << 16;
           float f;
           std::memcpy(&f, &w, sizeof(f));
