@@ -357,7 +357,10 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
 // Precisions (NPART):
 //  * 0: FP32 MFMA v_mfma_f32_16x16x4_f32, K stepped by 4.  Operand layout
 //    (wave64): lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16];
-//    D[4 (l / 16) + r][l % 16] = acc[r].  Offset matrices column-major.
+//    D[4 (l / 16) + r][l % 16] = acc[r].  Within a 16-wide k block, step j
+//    takes k = 4 (l / 16) + j, so a lane's four steps read four consecutive
+//    k: one 16-byte table load per row block (fragment order) and one
+//    16-byte LDS read per column block.
 //  * 2 or 3: BF16 MFMA v_mfma_f32_16x16x32_bf16, K stepped by 32; lane l
 //    supplies A[l % 16][8 (l / 16) .. + 7] and B[8 (l / 16) .. + 7][l % 16].
 //    Every FP32 operand x is split into BF16 parts x = x0 + x1 + x2 (each the
@@ -527,18 +530,27 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
     {
       if constexpr (NPART == 0) {
         const float *sF = reinterpret_cast<const float *>(s_raw);
-        const float *__restrict__ A = static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP;
-        const int row0 = 32 * wave + r16;
-#pragma unroll 4
-        for (int k0 = 0; k0 < KP; k0 += 4) {
-          const int k = k0 + kq;
-          const float a0 = A[k * KP + row0];
-          const float a1 = A[k * KP + row0 + 16];
+        const float4 *__restrict__ A =
+            reinterpret_cast<const float4 *>(static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP);
+#pragma unroll 1
+        for (int k16 = 0; k16 < KP / 16; ++k16) {
+          // steps j = 0..3 take k = 16 k16 + 4 kq + j (the tables' fragment order)
+          const float4 a0 = A[(k16 * (KP / 16) + 2 * wave) * 64 + lane];
+          const float4 a1 = A[(k16 * (KP / 16) + 2 * wave + 1) * 64 + lane];
+          float4 bv[4];
+#pragma unroll
+          for (int cb = 0; cb < 4; ++cb)
+            bv[cb] = *reinterpret_cast<const float4 *>(&sF[(r16 + 16 * cb) * LDF + 16 * k16 + 4 * kq]);
 #pragma unroll
           for (int cb = 0; cb < 4; ++cb) {
-            const float bv = sF[(r16 + 16 * cb) * LDF + k];
-            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, bv, acc[0][cb], 0, 0, 0);
-            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, bv, acc[1][cb], 0, 0, 0);
+            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, bv[cb].x, acc[0][cb], 0, 0, 0);
+            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, bv[cb].x, acc[1][cb], 0, 0, 0);
+            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, bv[cb].y, acc[0][cb], 0, 0, 0);
+            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, bv[cb].y, acc[1][cb], 0, 0, 0);
+            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, bv[cb].z, acc[0][cb], 0, 0, 0);
+            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, bv[cb].z, acc[1][cb], 0, 0, 0);
+            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, bv[cb].w, acc[0][cb], 0, 0, 0);
+            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, bv[cb].w, acc[1][cb], 0, 0, 0);
           }
         }
       } else {
@@ -1442,10 +1454,10 @@ int off_code(int x, int y, int z) { return ((x + 8) * 17 + (y + 8)) * 17 + (z + 
 struct FmmTables {
   int P = 0, KP = 0;
   int nslotF = 0, nslotH = 0;
-  DeviceBuffer T;         // [noff][KP][KP] column-major
+  DeviceBuffer T;         // [noff][KP x KP in FP32 fragment order]
   DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP x KP in fragment order]
-  DeviceBuffer W_m2m;     // [8][KP][KP]
-  DeviceBuffer W_l2l;     // [8][KP][KP]
+  DeviceBuffer W_m2m;     // [8][KP x KP in FP32 fragment order]
+  DeviceBuffer W_l2l;     // [8][KP x KP in FP32 fragment order]
   DeviceBuffer slot_tab;  // [kSlotRows][slot_ld] table index per slot (-1 unused)
   DeviceBuffer offsF;     // [8][nslotF] int3 (padding 0x7fffffff)
   DeviceBuffer offsH;     // [8][nslotH]
@@ -1521,6 +1533,16 @@ size_t bf16_frag_index(int KP, int p, int row, int col) {
   return piece * 8 + e;
 }
 
+// Position of element (row, col) within one FP32 operator table (M2L, M2M,
+// L2L).  MFMA step j of 16-wide k block k16 takes k = 16 k16 + 4 kq + j from
+// lane 16 kq + r16, so a lane's four steps are four consecutive k of its row:
+// one 16-byte load per 16-row block and k block, 1 KB per wave in lane order.
+size_t fp32_frag_index(int KP, int row, int col) {
+  const int k16 = col / 16, kq = (col % 16) / 4, j = col % 4;
+  const int rb = row / 16, r16 = row % 16;
+  return ((static_cast<size_t>(k16) * (KP / 16) + rb) * 64 + 16 * kq + r16) * 4 + j;
+}
+
 int kp_for(int P) {
   const int nd = (P + 1) * (P + 1);
   return ((nd + 31) / 32) * 32;
@@ -1579,12 +1601,14 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
       hoh[b * t->nslotH + k] = oh[b][k];
     }
   }
-  // M2L tables: FP32 column-major, and BF16 parts in fragment order
+  // M2L tables: FP32 and BF16 parts, both in fragment order
   std::vector<float> T(static_cast<size_t>(uniq.size()) * KP * KP);
   std::vector<uint16_t> Tb(static_cast<size_t>(uniq.size()) * 3 * KP * KP);
   for (size_t u = 0; u < uniq.size(); ++u) {
     const std::vector<double> m = m2l_matrix(P, KP, uniq[u].x, uniq[u].y, uniq[u].z);
-    for (size_t e = 0; e < m.size(); ++e) T[u * KP * KP + e] = static_cast<float>(m[e]);
+    for (int col = 0; col < KP; ++col)
+      for (int row = 0; row < KP; ++row)
+        T[u * KP * KP + fp32_frag_index(KP, row, col)] = static_cast<float>(m[static_cast<size_t>(col) * KP + row]);
     for (int row = 0; row < KP; ++row)
       for (int col = 0; col < KP; ++col) {
         double r = m[static_cast<size_t>(col) * KP + row];
@@ -1605,10 +1629,12 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
                  dz = (((oct >> 2) & 1) ? 0.25 : -0.25);
     const std::vector<double> a = m2m_matrix(P, KP, dx, dy, dz);
     const std::vector<double> b = l2l_matrix(P, KP, dx, dy, dz);
-    for (size_t e = 0; e < a.size(); ++e) {
-      W1[oct * KP * KP + e] = static_cast<float>(a[e]);
-      W2[oct * KP * KP + e] = static_cast<float>(b[e]);
-    }
+    for (int col = 0; col < KP; ++col)
+      for (int row = 0; row < KP; ++row) {
+        const size_t e = static_cast<size_t>(col) * KP + row, f = oct * KP * KP + fp32_frag_index(KP, row, col);
+        W1[f] = static_cast<float>(a[e]);
+        W2[f] = static_cast<float>(b[e]);
+      }
   }
   const std::vector<int3> nf = near_offsets_field(), nh = near_offsets_hess();
   if (static_cast<int>(nf.size()) != kNearF || static_cast<int>(nh.size()) != kNearH)
