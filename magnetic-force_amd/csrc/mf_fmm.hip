@@ -934,15 +934,18 @@ __device__ __forceinline__ double group_sum(double v) {
 }
 
 // Near-source stream of a sparse leaf (the T < 64 shapes).  The leaf's near
-// ranges, concatenated, are walked in windows of 64 records: lane l loads
-// record w0 + l (one coalesced load per window, issued while the previous
+// ranges, concatenated, are walked in windows of kStreamW records: lane l
+// loads records w0 + l + 64 r (coalesced loads, issued while the previous
 // window is evaluated), the window goes through LDS, and lane group g takes
 // records g, g + G, ... of it.  A range-by-range walk instead waits once per
 // G sources on a load, which dominated these leaves' time.
+constexpr int kStreamR = 2;              // records per lane per window
+constexpr int kStreamW = 64 * kStreamR;  // window length
+
 struct NearStream {
-  int pre[kNearH + 1];  // stream offset of each range; pre[nrange] = total
-  int beg[kNearH];      // first record of each range
-  float4 win[2][64];    // the window: record halves {x, y, z, -}, {mx, my, mz, -}
+  int pre[kNearH + 1];      // stream offset of each range; pre[nrange] = total
+  int beg[kNearH];          // first record of each range
+  float4 win[2][kStreamW];  // the window: record halves {x, y, z, -}, {mx, my, mz, -}
 };
 
 // Fills pre/beg from the leaf's ranges (one wave); returns the total.
@@ -982,16 +985,31 @@ __device__ __forceinline__ int near_stream_record(const NearStream &S, int nrang
   return S.beg[lo] + (i - S.pre[lo]);
 }
 
-// Loads the window starting at w0 into (a, b) (lane l: position w0 + l).
+// Loads the window starting at w0 into (a, b) (lane l: positions w0 + l + 64 r).
 __device__ __forceinline__ void near_stream_fetch(const NearStream &S, const SrcRec *__restrict__ rec,
                                                   int nrange, int total, int w0, int lane,
-                                                  float4 &a, float4 &b) {
-  const int i = w0 + lane;
-  if (i < total) {
-    const float4 *p = reinterpret_cast<const float4 *>(rec + near_stream_record(S, nrange, i));
-    a = p[0];
-    b = p[1];
+                                                  float4 (&a)[kStreamR], float4 (&b)[kStreamR]) {
+#pragma unroll
+  for (int r = 0; r < kStreamR; ++r) {
+    const int i = w0 + lane + 64 * r;
+    if (i < total) {
+      const float4 *p = reinterpret_cast<const float4 *>(rec + near_stream_record(S, nrange, i));
+      a[r] = p[0];
+      b[r] = p[1];
+    }
   }
+}
+
+// Publishes the fetched window in LDS (after the previous one is consumed).
+__device__ __forceinline__ void near_stream_publish(NearStream &S, int lane, const float4 (&a)[kStreamR],
+                                                    const float4 (&b)[kStreamR]) {
+  __syncthreads();  // the previous window is consumed
+#pragma unroll
+  for (int r = 0; r < kStreamR; ++r) {
+    S.win[0][lane + 64 * r] = a[r];
+    S.win[1][lane + 64 * r] = b[r];
+  }
+  __syncthreads();
 }
 
 // P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
@@ -1037,15 +1055,14 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z;
     double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
+    float4 wa[kStreamR], wb[kStreamR];
+#pragma unroll
+    for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
     near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
-    for (int w0 = 0; w0 < total; w0 += 64) {
-      __syncthreads();  // the previous window is consumed
-      S.win[0][lane] = wa;
-      S.win[1][lane] = wb;
-      __syncthreads();
-      if (w0 + 64 < total) near_stream_fetch(S, rec, nrange, total, w0 + 64, lane, wa, wb);
-      const int n = min(64, total - w0);
+    for (int w0 = 0; w0 < total; w0 += kStreamW) {
+      near_stream_publish(S, lane, wa, wb);
+      if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
+      const int n = min(kStreamW, total - w0);
       float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
       for (int k = g; k < n; k += G) {
         const float4 pa = S.win[0][k], pb = S.win[1][k];
@@ -1237,15 +1254,14 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
     double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
-    float4 wa = make_float4(0.f, 0.f, 0.f, 0.f), wb = wa;
+    float4 wa[kStreamR], wb[kStreamR];
+#pragma unroll
+    for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
     near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
-    for (int w0 = 0; w0 < total; w0 += 64) {
-      __syncthreads();  // the previous window is consumed
-      S.win[0][lane] = wa;
-      S.win[1][lane] = wb;
-      __syncthreads();
-      if (w0 + 64 < total) near_stream_fetch(S, rec, nrange, total, w0 + 64, lane, wa, wb);
-      const int n = min(64, total - w0);
+    for (int w0 = 0; w0 < total; w0 += kStreamW) {
+      near_stream_publish(S, lane, wa, wb);
+      if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
+      const int n = min(kStreamW, total - w0);
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
       for (int k = g; k < n; k += G) {
         const float4 pa = S.win[0][k], pb = S.win[1][k];
