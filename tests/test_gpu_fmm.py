@@ -108,6 +108,25 @@ def test_fmm_sparse_deep_tree(ctx, oracle):
     assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < 2e-6
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_fmm_lattice_under_sparse_layer(ctx, oracle, prec):
+    """A lattice slab under a sparse random layer (the pool's shape, at a size
+    the oracle covers).  The layer's leaves hold 1..16 particles and have
+    lattice leaves in their near sets, so the sparse-leaf kernels stream many
+    windows of sources, across near ranges of very different lengths."""
+    import magforce
+    lat = slab(24, 24, 8, seed=31)
+    rng = np.random.default_rng(32)
+    layer = rng.uniform(0.0, 1.0, size=(900, 3)) * np.array([24.0, 24.0, 8.0]) + np.array([0.0, 0.0, 8.0])
+    pos = f32(np.concatenate([lat, layer]))
+    m = f32(random_moments(pos.shape[0], seed=33))
+    code = {"fp32": magforce.M2L_FP32, "bf16x3": magforce.M2L_BF16X3}[prec]
+    tol = 2e-6 if prec == "fp32" else BF16_TOL["bf16x3"][10]
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, m2l_precision=code)
+    assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < tol
+    assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < tol
+
+
 def test_fmm_hconst_and_update_positions(ctx, oracle):
     import magforce
     pos = slab(20, 20, 6, seed=21)
