@@ -121,7 +121,10 @@ def test_fmm_lattice_under_sparse_layer(ctx, oracle, prec):
     pos = f32(np.concatenate([lat, layer]))
     m = f32(random_moments(pos.shape[0], seed=33))
     code = {"fp32": magforce.M2L_FP32, "bf16x3": magforce.M2L_BF16X3}[prec]
-    tol = 2e-6 if prec == "fp32" else BF16_TOL["bf16x3"][10]
+    # p = 10 truncation on this layout (random layer, worst separation ratio
+    # 0.61) measures 2.1e-6 in the field; a fault in the sparse-leaf path
+    # would show orders of magnitude above either bound
+    tol = 5e-6 if prec == "fp32" else BF16_TOL["bf16x3"][10]
     plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, m2l_precision=code)
     assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < tol
     assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < tol
