@@ -42,7 +42,10 @@ Status workspace(mf_ctx *ctx, const std::string &name, size_t bytes, void **out)
   return Status::Ok();
 }
 
-KernelTimer::KernelTimer(mf_ctx *ctx, const char *name) : ctx_(ctx), name_(name) {
+KernelTimer::KernelTimer(mf_ctx *ctx, const char *name) : KernelTimer(ctx, name, ctx->stream) {}
+
+KernelTimer::KernelTimer(mf_ctx *ctx, const char *name, hipStream_t stream)
+    : ctx_(ctx), name_(name), stream_(stream) {
   if (!ctx_->timing) return;
   auto take = [&]() {
     hipEvent_t e = nullptr;
@@ -56,12 +59,12 @@ KernelTimer::KernelTimer(mf_ctx *ctx, const char *name) : ctx_(ctx), name_(name)
   };
   start_ = take();
   stop_ = take();
-  if (start_ && stop_) (void)hipEventRecord(start_, ctx_->stream);
+  if (start_ && stop_) (void)hipEventRecord(start_, stream_);
 }
 
 void KernelTimer::stop() {
   if (!ctx_ || !start_ || !stop_) return;
-  (void)hipEventRecord(stop_, ctx_->stream);
+  (void)hipEventRecord(stop_, stream_);
   ctx_->timers[name_].pending.emplace_back(start_, stop_);
   start_ = stop_ = nullptr;
 }
@@ -199,11 +202,18 @@ int mf_ctx_create(int device, mf_ctx **out) {
       return Status::Err(MF_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(err));
     }
     ctx->stream = ctx->own_stream;
-    err = hipHostMalloc(reinterpret_cast<void **>(&ctx->host_scalars), sizeof(double) * 128);
+    err = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
+    if (err == hipSuccess)
+      err = hipHostMalloc(reinterpret_cast<void **>(&ctx->host_scalars), sizeof(double) * 128);
     if (err != hipSuccess) {
+      if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+      if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+      if (ctx->aux_stream) (void)hipStreamDestroy(ctx->aux_stream);
       (void)hipStreamDestroy(ctx->own_stream);
       delete ctx;
-      return Status::Err(MF_ERR_ALLOC, std::string("hipHostMalloc: ") + hipGetErrorString(err));
+      return Status::Err(MF_ERR_ALLOC, std::string("context set-up: ") + hipGetErrorString(err));
     }
     int cu = 0;
     if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
@@ -227,6 +237,12 @@ int mf_ctx_destroy(mf_ctx *ctx) {
   for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   ctx->ws.clear();
   if (ctx->host_scalars) (void)hipHostFree(ctx->host_scalars);
+  if (ctx->aux_stream) {
+    (void)hipStreamSynchronize(ctx->aux_stream);
+    (void)hipStreamDestroy(ctx->aux_stream);
+  }
+  if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+  if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
   if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return MF_OK;

@@ -36,6 +36,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -817,13 +818,16 @@ __global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_
   if (lane == 0) nr[b] = make_int2(n0, pos - n0);
 }
 
-// L2P, field: out[perm] = H_const + far field.  One wave per leaf.
+// L2P, field: out[perm] = H_const + far field, or out[perm] += far field
+// (accumulate; the near-field kernels have written H_const + near).  One
+// wave per leaf.
 template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_field_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const float *__restrict__ Lt, Frame fr, double s, float hc0,
-    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
+    float hc1, float hc2, int accumulate, float *__restrict__ ox, float *__restrict__ oy,
+    float *__restrict__ oz) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   double cx, cy, cz;
@@ -840,9 +844,12 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(
                 static_cast<float>(me.z - cz) * inv_s, gx, gy, gz);
     if (t < cnt) {
       const int o = perm[s0 + t];
-      ox[o] = static_cast<float>(hc0 + far_scale * gx);
-      oy[o] = static_cast<float>(hc1 + far_scale * gy);
-      oz[o] = static_cast<float>(hc2 + far_scale * gz);
+      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
+      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
+      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
+      ox[o] = static_cast<float>(b0 + far_scale * gx);
+      oy[o] = static_cast<float>(b1 + far_scale * gy);
+      oz[o] = static_cast<float>(b2 + far_scale * gz);
     }
   }
 }
@@ -1166,14 +1173,16 @@ struct Text9 {
   const float *p[9];
 };
 
-// L2P, force: F[perm] = (far tensor + T_ext) . m.  One wave per leaf.
+// L2P, force: F[perm] = (far tensor + T_ext) . m, or F[perm] += that
+// (accumulate; the near-field kernels have written the near force).  One
+// wave per leaf.
 template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_force_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const float *__restrict__ Lt, int use_far, Frame fr,
-    double s, Text9 text, int has_text, float *__restrict__ fx, float *__restrict__ fy,
-    float *__restrict__ fz) {
+    double s, Text9 text, int has_text, int accumulate, float *__restrict__ fx,
+    float *__restrict__ fy, float *__restrict__ fz) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   double cx, cy, cz;
@@ -1209,6 +1218,11 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
         F1 += v[1];
         F2 += v[2];
       }
+      if (accumulate) {
+        F0 += static_cast<double>(fx[o]);
+        F1 += static_cast<double>(fy[o]);
+        F2 += static_cast<double>(fz[o]);
+      }
       fx[o] = static_cast<float>(F0);
       fy[o] = static_cast<float>(F1);
       fz[o] = static_cast<float>(F2);
@@ -1216,14 +1230,14 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
   }
 }
 
-// P2P, force: F[perm] += (sum over the leaf's near ranges of B[m_j]) . m_i.
+// P2P, force: F[perm] (+)= (sum over the leaf's near ranges of B[m_j]) . m_i.
 // Leaf shapes (T) as in p2p_field_kernel.
 template <int T>
 __global__ __launch_bounds__(64) void p2p_force_kernel(
     const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
-    int nslot, HessConsts hc, float *__restrict__ fx, float *__restrict__ fy,
+    int nslot, HessConsts hc, int accumulate, float *__restrict__ fx, float *__restrict__ fy,
     float *__restrict__ fz) {
   constexpr int G = 64 / T;
   const int li = xcd_contiguous(nleaves);
@@ -1302,9 +1316,9 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     f2 = group_sum<T>(f2);
     if (g == 0 && t < te) {
       const int o = perm[s0 + t];
-      fx[o] = static_cast<float>(static_cast<double>(fx[o]) + sfac * f0);
-      fy[o] = static_cast<float>(static_cast<double>(fy[o]) + sfac * f1);
-      fz[o] = static_cast<float>(static_cast<double>(fz[o]) + sfac * f2);
+      fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
+      fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
+      fz[o] = static_cast<float>((accumulate ? static_cast<double>(fz[o]) : 0.0) + sfac * f2);
     }
     return;
   }
@@ -1358,9 +1372,9 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     }
     if (g == 0 && t < te) {
       const int o = perm[s0 + t];
-      fx[o] = static_cast<float>(static_cast<double>(fx[o]) + sfac * f0);
-      fy[o] = static_cast<float>(static_cast<double>(fy[o]) + sfac * f1);
-      fz[o] = static_cast<float>(static_cast<double>(fz[o]) + sfac * f2);
+      fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
+      fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
+      fz[o] = static_cast<float>((accumulate ? static_cast<double>(fz[o]) : 0.0) + sfac * f2);
     }
   }
 }
@@ -2179,14 +2193,18 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
   return Status::Ok();
 }
 
+// Upward pass, M2L and L2L on the context stream.  `before_m2l` runs once the
+// upward pass is issued, right before the M2L launches: the products launch
+// their near field there, on the auxiliary stream, so that it shares the GPU
+// with the M2L (MFMA) rather than with the P2M (which is LDS-bound).
 template <int P, int KP>
-Status upward_and_m2l(mf_plan *plan, bool hess) {
+Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &before_m2l) {
   mf_ctx *ctx = plan->ctx;
   FmmState &F = *plan->fmm;
   const FmmTables &tab = *F.tab;
   const int D = F.D;
   const int nleaf = F.count[D];
-  if (D < 2) return Status::Ok();
+  if (D < 2) return before_m2l();
   {
     KernelTimer timer(ctx, "p2m");
     hipLaunchKernelGGL((p2m_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
@@ -2211,16 +2229,18 @@ Status upward_and_m2l(mf_plan *plan, bool hess) {
       MF_LAUNCH_CHECK();
     }
   }
+  const int prec = plan->m2l_precision();
+  if (prec != MF_M2L_FP32) {
+    // split every multipole once; the M2L gathers each one many times
+    KernelTimer timer(ctx, "split");
+    const int64_t ne = static_cast<int64_t>(F.nbox) * KP;
+    hipLaunchKernelGGL(split_multipoles_kernel, dim3(grid_for(ne, 256)), dim3(256), 0,
+                       ctx->stream, ne, KP, F.Mt.as<float>(), F.Mb.as<uint16_t>());
+    MF_LAUNCH_CHECK();
+  }
+  MF_TRY(before_m2l());
   {
     KernelTimer timer(ctx, hess ? "m2l_force" : "m2l");
-    const int prec = plan->m2l_precision();
-    if (prec != MF_M2L_FP32) {
-      // split every multipole once; the M2L gathers each one many times
-      const int64_t ne = static_cast<int64_t>(F.nbox) * KP;
-      hipLaunchKernelGGL(split_multipoles_kernel, dim3(grid_for(ne, 256)), dim3(256), 0,
-                         ctx->stream, ne, KP, F.Mt.as<float>(), F.Mb.as<uint16_t>());
-      MF_LAUNCH_CHECK();
-    }
     if (!hess) {
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
                             tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>()));
@@ -2272,37 +2292,49 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   hipLaunchKernelGGL(pack_sorted_moments_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream,
                      n, F.perm.as<int32_t>(), px, py, pz, scale, plan->weights, F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
-  MF_TRY((upward_and_m2l<P, KP>(plan, false)));
   const int D = F.D, nleaf = F.count[D];
   const float h0 = hconst ? hconst[0] : 0.f, h1 = hconst ? hconst[1] : 0.f,
               h2 = hconst ? hconst[2] : 0.f;
   const int use_far = D >= 2 ? 1 : 0;
+  {
+    // The near field (VALU) runs on the auxiliary stream, concurrently with
+    // the M2L (MFMA); it writes H_const + near, and the L2P adds the far
+    // field once both are done.
+    NearOverlap ov(ctx);
+    auto near = [&]() -> Status {
+      MF_TRY(ov.fork());
+      KernelTimer timer(ctx, "p2p_field", ctx->aux_stream);
+      const int32_t *lists = F.leaf_lists.as<int32_t>();
+#define MF_P2P_FIELD(T, k)                                                                       \
+  if (F.leaf_list_n[k] > 0) {                                                                    \
+    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+                       0, ctx->aux_stream, lists + F.leaf_list_off[k],                         \
+                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
+                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
+                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc, h0, h1, h2,    \
+                       ox, oy, oz);                                                              \
+    MF_LAUNCH_CHECK();                                                                           \
+  }
+      MF_P2P_FIELD(64, 0)
+      MF_P2P_FIELD(32, 1)
+      MF_P2P_FIELD(16, 2)
+      MF_P2P_FIELD(8, 3)
+#undef MF_P2P_FIELD
+      timer.stop();
+      return ov.record();
+    };
+    MF_TRY((upward_and_m2l<P, KP>(plan, false, near)));
+    MF_TRY(ov.join());
+  }
   if (use_far) {
     KernelTimer timer(ctx, "l2p_field");
     hipLaunchKernelGGL((l2p_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
                        F.Lt.as<float>(), Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, h0, h1, h2,
-                       ox, oy, oz);
+                       1, ox, oy, oz);
     MF_LAUNCH_CHECK();
   }
-  KernelTimer timer(ctx, "p2p_field");
-  const int32_t *lists = F.leaf_lists.as<int32_t>();
-#define MF_P2P_FIELD(T, k)                                                                       \
-  if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
-                       0, ctx->stream, lists + F.leaf_list_off[k],                             \
-                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
-                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
-                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, use_far, plan->fc, h0, h1,  \
-                       h2, ox, oy, oz);                                                          \
-    MF_LAUNCH_CHECK();                                                                           \
-  }
-  MF_P2P_FIELD(64, 0)
-  MF_P2P_FIELD(32, 1)
-  MF_P2P_FIELD(16, 2)
-  MF_P2P_FIELD(8, 3)
-#undef MF_P2P_FIELD
   return Status::Ok();
 }
 
@@ -2317,7 +2349,34 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
                      n, F.perm.as<int32_t>(), mx, my, mz, 1.f, static_cast<const float *>(nullptr),
                      F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
-  MF_TRY((upward_and_m2l<P, KP>(plan, true)));
+  {
+    // near force on the auxiliary stream (writes), concurrently with the
+    // M2L; the L2P adds the far part and T_ext . m
+    NearOverlap ov(ctx);
+    auto near = [&]() -> Status {
+      MF_TRY(ov.fork());
+      KernelTimer timer(ctx, "p2p_force", ctx->aux_stream);
+      const int32_t *lists = F.leaf_lists.as<int32_t>();
+#define MF_P2P_FORCE(T, k)                                                                       \
+  if (F.leaf_list_n[k] > 0) {                                                                    \
+    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+                       0, ctx->aux_stream, lists + F.leaf_list_off[k],                         \
+                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
+                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
+                       F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, 0, fx, fy, fz);  \
+    MF_LAUNCH_CHECK();                                                                           \
+  }
+      MF_P2P_FORCE(64, 0)
+      MF_P2P_FORCE(32, 1)
+      MF_P2P_FORCE(16, 2)
+      MF_P2P_FORCE(8, 3)
+#undef MF_P2P_FORCE
+      timer.stop();
+      return ov.record();
+    };
+    MF_TRY((upward_and_m2l<P, KP>(plan, true, near)));
+    MF_TRY(ov.join());
+  }
   Text9 t{};
   int has_text = 0;
   if (text9) {
@@ -2334,25 +2393,9 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
                        F.Lt.as<float>(), D >= 2 ? 1 : 0, Frame{F.sys_lo.as<double>(), 3 * D},
-                       F.s_leaf, t, has_text, fx, fy, fz);
+                       F.s_leaf, t, has_text, 1, fx, fy, fz);
     MF_LAUNCH_CHECK();
   }
-  KernelTimer timer(ctx, "p2p_force");
-  const int32_t *lists = F.leaf_lists.as<int32_t>();
-#define MF_P2P_FORCE(T, k)                                                                       \
-  if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
-                       0, ctx->stream, lists + F.leaf_list_off[k],                             \
-                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
-                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
-                       F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, fx, fy, fz);     \
-    MF_LAUNCH_CHECK();                                                                           \
-  }
-  MF_P2P_FORCE(64, 0)
-  MF_P2P_FORCE(32, 1)
-  MF_P2P_FORCE(16, 2)
-  MF_P2P_FORCE(8, 3)
-#undef MF_P2P_FORCE
   return Status::Ok();
 }
 

@@ -115,6 +115,11 @@ struct mf_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
+  // Second stream (non-blocking, default priority) on which the FMM products
+  // run their near field beside the far field; forked from and joined to
+  // `stream` by these events within every product.
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string last_error;
   bool timing = false;
   std::map<std::string, mf::TimingRecord> timers;
@@ -133,17 +138,59 @@ namespace mf {
 // Returns a device workspace of at least `bytes` for `name`.
 Status workspace(mf_ctx *ctx, const std::string &name, size_t bytes, void **out);
 
-// Records timing events around a kernel family when timing is enabled.
+// Records timing events around a kernel family when timing is enabled, on
+// the stream the family is launched on (the context's stream by default).
 class KernelTimer {
  public:
   KernelTimer(mf_ctx *ctx, const char *name);
+  KernelTimer(mf_ctx *ctx, const char *name, hipStream_t stream);
   ~KernelTimer();
   void stop();
 
  private:
   mf_ctx *ctx_;
   const char *name_;
+  hipStream_t stream_ = nullptr;
   hipEvent_t start_ = nullptr, stop_ = nullptr;
+};
+
+// Fork/join of the context's auxiliary stream around one product: fork()
+// makes it wait for the work issued so far on the context stream, record()
+// marks the end of the work issued on it, join() makes the context stream
+// wait for that.  The destructor joins on paths that return early, so work
+// on the auxiliary stream never overtakes the context stream's next use of
+// the same buffers.  Inside a stream capture the two events pull the
+// auxiliary stream into the graph and back.
+class NearOverlap {
+ public:
+  explicit NearOverlap(mf_ctx *ctx) : ctx_(ctx) {}
+  NearOverlap(const NearOverlap &) = delete;
+  NearOverlap &operator=(const NearOverlap &) = delete;
+  ~NearOverlap() {
+    if (!forked_ || joined_) return;
+    if (!recorded_) (void)hipEventRecord(ctx_->ev_join, ctx_->aux_stream);
+    (void)hipStreamWaitEvent(ctx_->stream, ctx_->ev_join, 0);
+  }
+  Status fork() {
+    MF_HIP(hipEventRecord(ctx_->ev_fork, ctx_->stream));
+    MF_HIP(hipStreamWaitEvent(ctx_->aux_stream, ctx_->ev_fork, 0));
+    forked_ = true;
+    return Status::Ok();
+  }
+  Status record() {
+    MF_HIP(hipEventRecord(ctx_->ev_join, ctx_->aux_stream));
+    recorded_ = true;
+    return Status::Ok();
+  }
+  Status join() {
+    MF_HIP(hipStreamWaitEvent(ctx_->stream, ctx_->ev_join, 0));
+    joined_ = true;
+    return Status::Ok();
+  }
+
+ private:
+  mf_ctx *ctx_;
+  bool forked_ = false, recorded_ = false, joined_ = false;
 };
 
 // Collects finished event pairs into the totals (synchronizes on them).
