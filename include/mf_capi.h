@@ -158,7 +158,10 @@ typedef struct mf_plan_opts {
   int leaf_size;     /* unused (the octree leaf side is fixed at 2h); must be 0 */
   int order;         /* FMM: expansion order p (degrees 0..p): 6, 8 or 10; 0 = 10 */
   int m2l_precision; /* FMM: mf_m2l_precision of mf_plan_field / mf_plan_force */
-  int reserved[4];
+  int serial_near;   /* FMM: 0 = the near field runs on a second stream beside the M2L
+                        (default); 1 = on the context stream, after the upward pass (for
+                        timing each kernel alone) */
+  int reserved[3];
 } mf_plan_opts;
 
 /* Replaces the construction of fmm_field_evaluator / fmm_hessian_evaluator

@@ -466,6 +466,10 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
     }
+    if (plan->opts.serial_near != 0 && plan->opts.serial_near != 1) {
+      delete plan;
+      return Status::Err(MF_ERR_INVALID_ARG, "serial_near must be 0 or 1");
+    }
     plan->h = h;
     plan->fc = mf::make_field_consts(h);
     plan->hc = mf::make_hess_consts(h);
@@ -494,6 +498,7 @@ int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const fl
     MF_ARG(h > 0.f, "h must be > 0");
     MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X3,
            "unknown M2L precision");
+    MF_ARG(opts->serial_near == 0 || opts->serial_near == 1, "serial_near must be 0 or 1");
     MF_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<mf_plan> plan(new (std::nothrow) mf_plan());
     if (!plan) return Status::Err(MF_ERR_ALLOC, "out of host memory");

@@ -2301,14 +2301,19 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
     // the M2L (MFMA); it writes H_const + near, and the L2P adds the far
     // field once both are done.
     NearOverlap ov(ctx);
+    const bool serial = plan->opts.serial_near != 0;
     auto near = [&]() -> Status {
-      MF_TRY(ov.fork());
-      KernelTimer timer(ctx, "p2p_field", ctx->aux_stream);
+      hipStream_t ns = ctx->stream;
+      if (!serial) {
+        MF_TRY(ov.fork());
+        ns = ctx->aux_stream;
+      }
+      KernelTimer timer(ctx, "p2p_field", ns);
       const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FIELD(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
     hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
-                       0, ctx->aux_stream, lists + F.leaf_list_off[k],                         \
+                       0, ns, lists + F.leaf_list_off[k],                                      \
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
                        F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc, h0, h1, h2,    \
@@ -2321,7 +2326,7 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       MF_P2P_FIELD(8, 3)
 #undef MF_P2P_FIELD
       timer.stop();
-      return ov.record();
+      return serial ? Status::Ok() : ov.record();
     };
     MF_TRY((upward_and_m2l<P, KP>(plan, false, near)));
     MF_TRY(ov.join());
@@ -2353,14 +2358,19 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
     // near force on the auxiliary stream (writes), concurrently with the
     // M2L; the L2P adds the far part and T_ext . m
     NearOverlap ov(ctx);
+    const bool serial = plan->opts.serial_near != 0;
     auto near = [&]() -> Status {
-      MF_TRY(ov.fork());
-      KernelTimer timer(ctx, "p2p_force", ctx->aux_stream);
+      hipStream_t ns = ctx->stream;
+      if (!serial) {
+        MF_TRY(ov.fork());
+        ns = ctx->aux_stream;
+      }
+      KernelTimer timer(ctx, "p2p_force", ns);
       const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FORCE(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
     hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
-                       0, ctx->aux_stream, lists + F.leaf_list_off[k],                         \
+                       0, ns, lists + F.leaf_list_off[k],                                      \
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
                        F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, 0, fx, fy, fz);  \
@@ -2372,7 +2382,7 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       MF_P2P_FORCE(8, 3)
 #undef MF_P2P_FORCE
       timer.stop();
-      return ov.record();
+      return serial ? Status::Ok() : ov.record();
     };
     MF_TRY((upward_and_m2l<P, KP>(plan, true, near)));
     MF_TRY(ov.join());

@@ -183,6 +183,7 @@ class NearOverlap {
     return Status::Ok();
   }
   Status join() {
+    if (!forked_) return Status::Ok();
     MF_HIP(hipStreamWaitEvent(ctx_->stream, ctx_->ev_join, 0));
     joined_ = true;
     return Status::Ok();

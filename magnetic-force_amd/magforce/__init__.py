@@ -54,7 +54,8 @@ class MagforceError(RuntimeError):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("leaf_size", ctypes.c_int), ("order", ctypes.c_int),
-                ("m2l_precision", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("m2l_precision", ctypes.c_int), ("serial_near", ctypes.c_int),
+                ("reserved", ctypes.c_int * 3)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -351,13 +352,14 @@ class Plan:
     octree (mode=MODE_FMM)."""
 
     def __init__(self, ctx: Context, pos, h: float, mode: int = MODE_ALLPAIRS,
-                 leaf_size: int = 0, order: int = 0, m2l_precision: int = 0):
+                 leaf_size: int = 0, order: int = 0, m2l_precision: int = 0,
+                 serial_near: bool = False):
         self.ctx = ctx
         self.h = float(h)
         p = _soa(pos)
         self.n = p.shape[1]
         self.opts = PlanOpts(mode=mode, leaf_size=leaf_size, order=order,
-                             m2l_precision=m2l_precision)
+                             m2l_precision=m2l_precision, serial_near=int(bool(serial_near)))
         hp = ctypes.c_void_p()
         _check(lib().mf_plan_create(ctx.h, self.n, _ptr(p, 0), _ptr(p, 1), _ptr(p, 2), self.h,
                                     ctypes.byref(self.opts), ctypes.byref(hp)), ctx.h)
@@ -446,7 +448,7 @@ class BatchPlan(Plan):
     block-diagonal; solve() runs the batched CG with one kappa per system."""
 
     def __init__(self, ctx: Context, pos, offsets, h: float, order: int = 0,
-                 m2l_precision: int = 0):
+                 m2l_precision: int = 0, serial_near: bool = False):
         self.ctx = ctx
         self.h = float(h)
         p = _soa(pos)
@@ -456,7 +458,8 @@ class BatchPlan(Plan):
             raise ValueError("offsets must start at 0 and end at n")
         self.offsets = off
         self.nsys = len(off) - 1
-        self.opts = PlanOpts(mode=MODE_FMM, order=order, m2l_precision=m2l_precision)
+        self.opts = PlanOpts(mode=MODE_FMM, order=order, m2l_precision=m2l_precision,
+                             serial_near=int(bool(serial_near)))
         arr = (ctypes.c_int64 * len(off))(*off)
         hp = ctypes.c_void_p()
         _check(lib().mf_plan_create_batch(ctx.h, self.nsys, arr, _ptr(p, 0), _ptr(p, 1),

@@ -38,8 +38,9 @@ def test_null_arguments_are_rejected_without_a_device():
 
 
 def test_struct_layouts_match_header():
-    # mf_plan_opts: 4 ints + 4 reserved; mf_solve_opts: 3 ints, double, 4 ints
+    # mf_plan_opts: 5 ints + 3 reserved; mf_solve_opts: 3 ints, double, 4 ints
     assert ctypes.sizeof(magforce.PlanOpts) == 32
+    assert magforce.PlanOpts.serial_near.offset == 16
     assert ctypes.sizeof(magforce.SolveOpts) == 40
     assert ctypes.sizeof(magforce.SolveStats) == 32
 

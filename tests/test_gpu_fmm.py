@@ -130,6 +130,26 @@ def test_fmm_lattice_under_sparse_layer(ctx, oracle, prec):
     assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < tol
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_near_overlap_matches_serial(ctx, slab_case, prec):
+    """The near field on the second stream (the default) and serialized on the
+    context stream (serial_near) run the same kernels on the same data, so
+    their products are bitwise equal.  Products with different moments are
+    issued back to back without host synchronization: a near-field kernel
+    that ran past its join would read the next product's moments."""
+    import magforce
+    pos, m, _, _ = slab_case
+    code = {"fp32": magforce.M2L_FP32, "bf16x3": magforce.M2L_BF16X3}[prec]
+    a = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, m2l_precision=code)
+    b = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, m2l_precision=code,
+                      serial_near=True)
+    ms = [dev(m), dev(f32(random_moments(pos.shape[0], seed=41))), dev(-m)]
+    ga = [a.field(x) for x in ms] + [a.force(x) for x in ms]
+    gb = [b.field(x) for x in ms] + [b.force(x) for x in ms]
+    for u, v in zip(ga, gb):
+        assert np.array_equal(host(u), host(v))
+
+
 def test_fmm_hconst_and_update_positions(ctx, oracle):
     import magforce
     pos = slab(20, 20, 6, seed=21)
