@@ -839,14 +839,19 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(
   for (int t0 = 0; t0 < cnt; t0 += 64) {
     const int t = t0 + lane;
     const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
+    // the near-field values are read before the expansion is evaluated, so
+    // the scattered loads overlap the arithmetic
+    const int o = perm[s0 + (t < cnt ? t : cnt - 1)];
+    double b0 = hc0, b1 = hc1, b2 = hc2;
+    if (accumulate) {
+      b0 = static_cast<double>(ox[o]);
+      b1 = static_cast<double>(oy[o]);
+      b2 = static_cast<double>(oz[o]);
+    }
     float gx, gy, gz;
     l2p_grad<P>(L, static_cast<float>(me.x - cx) * inv_s, static_cast<float>(me.y - cy) * inv_s,
                 static_cast<float>(me.z - cz) * inv_s, gx, gy, gz);
     if (t < cnt) {
-      const int o = perm[s0 + t];
-      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
-      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
-      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
       ox[o] = static_cast<float>(b0 + far_scale * gx);
       oy[o] = static_cast<float>(b1 + far_scale * gy);
       oz[o] = static_cast<float>(b2 + far_scale * gz);
