@@ -13,6 +13,9 @@ Printed JSON line (rank 0): metric/value/unit per the contract, plus
                   the 262,144-particle cylinder,
   "roofline"    : dominant kernel, algorithmic FLOPs per launch / measured
                   launch time against the FP32 peak,
+  "timestep"    : the moving-particle step (plan rebuilt in place, solve
+                  warm-started from the last solution): ms/step, rebuild
+                  time and CG iterations warm against cold,
   "cpu_baseline": the CPU oracle timed on a bounded sample of the workload.
 """
 from __future__ import annotations
@@ -64,6 +67,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-allpairs", action="store_true")
+    p.add_argument("--no-timestep", action="store_true",
+                   help="skip the moving-particle time-step leg (warm-started solves)")
     p.add_argument("--profile-only", action="store_true",
                    help="only the timed steps (for rocprofv3 runs)")
     p.add_argument("--copy-calib-mb", type=int, default=0,
@@ -245,6 +250,67 @@ def check_error_subsample(ctx, st: Stepper, samples: int):
            "residual_rel_l2": res, "x_err_bound": (1.0 + s.chi) * res, "samples": k,
            "ref_seconds": time.perf_counter() - t0}
     log(f"check: {out}")
+    return out
+
+
+def run_timestep(ctx, st: Stepper, steps: int, sigma_h: float = 0.02, seed: int = 31):
+    """The simulation caller's time step (magnetic_force.cu:806-888 with the
+    unknown kept across Resize, :706-744): particles move a little, the plan
+    is rebuilt in place for the new positions, and the solve starts from the
+    previous step's solution.  Each step moves every particle by a normal
+    displacement of sigma_h * h per axis.  The displaced position sets are
+    made before the timed region; the timed step is update_positions + warm
+    CG to the same rtol + force.  After the loop the last positions are solved
+    again from zero, so the warm and cold solutions and iteration counts
+    compare on the same system."""
+    import torch
+    s = st.s
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    poses, p = [], st.pos.clone()
+    for _ in range(steps + 1):
+        p = p + torch.randn(p.shape, generator=gen, device="cuda") * (sigma_h * s.h)
+        poses.append(p)
+    x = st.x.clone()
+    kappa = s.kappa
+
+    def step(pos):
+        nonlocal x
+        st.plan.update_positions(pos)
+        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol)
+        x = res.x
+        st.plan.force(kappa * x, out=st.F)
+        return res.stats
+
+    step(poses[0])      # warm-up at the first displaced positions
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps + 2)]
+    iters = []
+    ev[0].record()
+    for k in range(steps):
+        ev[2 * k + 1].record()
+        st.plan.update_positions(poses[k + 1])
+        ev[2 * k + 2].record()
+        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol)
+        x = res.x
+        st.plan.force(kappa * x, out=st.F)
+        iters.append(res.stats)
+    ev[-1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[-1]) / steps
+    rebuild = [ev[2 * k + 1].elapsed_time(ev[2 * k + 2]) for k in range(steps)]
+    cold = st.plan.solve(kappa, st.b, x0=None, rtol=st.rtol)
+    torch.cuda.synchronize()
+    dx = float(torch.linalg.norm(x - cold.x) / torch.linalg.norm(cold.x))
+    out = {"motion": f"N(0, ({sigma_h} h)^2) per axis per step", "steps": steps,
+           "ms_per_step": ms, "rebuild_ms": sum(rebuild) / steps,
+           "iterations_warm": [int(r["iterations"]) for r in iters],
+           "matvecs_warm": [int(r["matvecs"]) for r in iters],
+           "iterations_cold": int(cold.stats["iterations"]),
+           "matvecs_cold": int(cold.stats["matvecs"]),
+           "warm_vs_cold_x_rel_l2": dx}
+    st.pos, st.x = poses[-1], x
+    log(f"timestep: {out}")
     return out
 
 
@@ -567,6 +633,9 @@ def main():
             check = check_error_subsample(ctx, st, args.check_samples or 65536)
         else:
             check = check_error(ctx, st, args.check_samples or 4096)
+    timestep = None
+    if mode == magforce.MODE_FMM and not args.no_timestep:
+        timestep = run_timestep(ctx, st, max(args.steps, 3))
     cpu = None if args.no_cpu else cpu_baseline(system, args.cpu_seconds, matvecs)
 
     line = {
@@ -585,6 +654,7 @@ def main():
         "batch": batch,
         "c1_lattice8k": c1,
         "allpairs": allpairs,
+        "timestep": timestep,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

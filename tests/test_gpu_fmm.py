@@ -181,6 +181,34 @@ def test_fmm_solve_and_force_vs_oracle(ctx, oracle):
     assert rel_err(F, F_ref) < 1e-5
 
 
+def test_fmm_time_step_warm_start(ctx, oracle):
+    """The simulation caller's step (magnetic_force.cu:806-888): the particles
+    of the 8192 lattice move by N(0, (0.02 h)^2) per axis, the plan is rebuilt
+    in place, and CG starts from the previous solution (the unknown kept
+    across Resize, :706-744).  The warm solve meets the same FP64 solution at
+    the moved positions as a cold one, in fewer iterations."""
+    import magforce
+    from magforce import configs
+    s = configs.lattice8k()
+    pos0 = s.pos.astype(np.float64)
+    b = np.tile(s.hconst, (s.n, 1))
+    plan = magforce.Plan(ctx, dev(pos0), s.h, mode=magforce.MODE_FMM)
+    x0 = plan.solve(s.kappa, dev(b), rtol=1e-6).x
+    rng = np.random.default_rng(41)
+    pos1 = f32(pos0 + rng.normal(scale=0.02 * s.h, size=pos0.shape))
+    plan.update_positions(dev(pos1))
+    warm = plan.solve(s.kappa, dev(b), x0=x0, rtol=1e-6)
+    cold = plan.solve(s.kappa, dev(b), rtol=1e-6)
+    assert warm.stats["converged"] == 1 and cold.stats["converged"] == 1
+    # FP64 CG on this system: 5 iterations warm, 7 cold
+    assert warm.stats["iterations"] < cold.stats["iterations"]
+    x_ref, _, _ = oracle.cg_solve(pos1, s.h, s.kappa, b, rtol=1e-12)
+    assert rel_err(host(warm.x), x_ref) < 5e-6
+    F = host(plan.force(dev(s.kappa * host(warm.x))))
+    F_ref = oracle.force_allpairs(pos1, s.kappa * x_ref, s.h)
+    assert rel_err(F, F_ref) < 1e-5
+
+
 def test_fmm_matches_fp64_gpu_at_100k(ctx):
     """A 100k-particle slab with a random layer above it, against the GPU FP64
     all-pairs field (itself pinned to the CPU oracle in test_gpu_allpairs)."""
