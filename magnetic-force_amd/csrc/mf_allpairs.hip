@@ -534,7 +534,71 @@ FieldConsts make_field_consts(double h) {
   const double g[7] = {1843.0 / 640.0, 27.0 / 32.0, -63.0 / 32.0, 5.0 / 4.0,
                        9.0 / 8.0,      -9.0 / 10.0, -1.0 / 2.0};
   for (int k = 0; k < 7; ++k) fc.g[k] = static_cast<float>(g[k]);
+  fc.tab_scale = static_cast<float>(kFieldTabN / (2.0 * h));
   return fc;
+}
+
+namespace {
+// alpha h^5 and beta h^3 of a near pair at q < 2, in FP64 (the forms of the
+// mf_pair.hpp header; g unexpanded, which is accurate in FP64).
+void near_coeffs_f64(double q, double &a, double &b) {
+  if (q < 1.0) {
+    a = 0.8 - 0.5 * q;
+    b = -8.0 / 3.0 + q * q * (4.8 - 2.5 * q);
+    return;
+  }
+  const double q3 = q * q * q;
+  const double g = -3.0 * (q3 * (((q / 6.0 - 1.2) * q + 3.0) * q - 8.0 / 3.0) + 1.0 / 15.0);
+  const double w = (2.0 - q) * (2.0 - q) * (2.0 - q);
+  a = (g / q3 - w) / (3.0 * q * q);
+  b = g / (3.0 * q3) - w;
+}
+
+// Power-basis coefficients c[0..3] of the cubic through (t[i], y[i]), i < 4
+// (Gaussian elimination with partial pivoting on the Vandermonde system).
+void cubic_through(const double (&t)[4], const double (&y)[4], double (&c)[4]) {
+  double m[4][5];
+  for (int i = 0; i < 4; ++i) {
+    double p = 1.0;
+    for (int j = 0; j < 4; ++j, p *= t[i]) m[i][j] = p;
+    m[i][4] = y[i];
+  }
+  for (int col = 0; col < 4; ++col) {
+    int piv = col;
+    for (int i = col + 1; i < 4; ++i)
+      if (std::fabs(m[i][col]) > std::fabs(m[piv][col])) piv = i;
+    for (int j = 0; j < 5; ++j) std::swap(m[col][j], m[piv][j]);
+    for (int i = col + 1; i < 4; ++i) {
+      const double f = m[i][col] / m[col][col];
+      for (int j = col; j < 5; ++j) m[i][j] -= f * m[col][j];
+    }
+  }
+  for (int i = 3; i >= 0; --i) {
+    double s = m[i][4];
+    for (int j = i + 1; j < 4; ++j) s -= m[i][j] * c[j];
+    c[i] = s / m[i][i];
+  }
+}
+}  // namespace
+
+void make_field_table(double h, float *tab) {
+  const double ih3 = 1.0 / (h * h * h), ih5 = ih3 / (h * h);
+  double t[4];
+  for (int i = 0; i < 4; ++i) t[i] = 0.5 * std::cos((2 * i + 1) * kPi / 8.0);
+  for (int k = 0; k < kFieldTabN; ++k) {
+    double ya[4], yb[4], ca[4], cb[4];
+    for (int i = 0; i < 4; ++i) {
+      near_coeffs_f64((k + 0.5 + t[i]) * 2.0 / kFieldTabN, ya[i], yb[i]);
+      ya[i] *= ih5;
+      yb[i] *= ih3;
+    }
+    cubic_through(t, ya, ca);
+    cubic_through(t, yb, cb);
+    for (int j = 0; j < 4; ++j) {
+      tab[8 * k + j] = static_cast<float>(ca[j]);
+      tab[8 * k + 4 + j] = static_cast<float>(cb[j]);
+    }
+  }
 }
 
 // Fitted curve table of the reference (Dipole_hessian.kern:53-61), highest

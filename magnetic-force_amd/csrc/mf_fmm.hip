@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <functional>
 #include <map>
@@ -890,9 +891,9 @@ __device__ __forceinline__ void field_range_far(const SrcRec *__restrict__ rec, 
 template <int STRIDE>
 __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec, int j0, int j1,
                                                   float tx, float ty, float tz,
-                                                  const FieldConsts &fc, float &a1x, float &a1y,
-                                                  float &a1z, float &a2x, float &a2y,
-                                                  float &a2z) {
+                                                  const FieldConsts &fc, const float4 *tab,
+                                                  float &a1x, float &a1y, float &a1z, float &a2x,
+                                                  float &a2y, float &a2z) {
 #pragma unroll 2
   for (int j = j0; j < j1; j += STRIDE) {
     const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
@@ -903,7 +904,7 @@ __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec
     const float ir2 = ir * ir;
     float beta = ir2 * ir;
     float alpha = beta * ir2;
-    if (r2 < fc.near2) field_near_coeffs(fc, r2, ir, alpha, beta);
+    if (r2 < fc.near2) field_near_coeffs_tab(tab, fc.tab_scale, r2, ir, alpha, beta);
     const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
     const float s1 = dm * alpha;
     a1x = fmaf(dx, s1, a1x);
@@ -1042,13 +1043,21 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
     const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
-    int nslot, int accumulate, FieldConsts fc, float hc0, float hc1, float hc2,
-    float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
+    int nslot, int accumulate, FieldConsts fc, const float4 *__restrict__ ftab, float hc0,
+    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
   constexpr int G = 64 / T;
   const int li = xcd_contiguous(nleaves);
   if (li < 0) return;
   const int b = leaves[li];
   const int lane = threadIdx.x;
+  // The full-leaf shape takes its near-pair coefficients from the table
+  // (2 KB of LDS).  The sparse shapes keep the closed form: their waves wait
+  // on source windows, and the extra LDS would cost them occupancy.
+  __shared__ float4 s_ftab[T == 64 ? 2 * kFieldTabN : 1];
+  if constexpr (T == 64) {
+    for (int i = lane; i < 2 * kFieldTabN; i += 64) s_ftab[i] = ftab[i];
+    __syncthreads();
+  }
   const int tl = T == 64 ? lane : lane % T;
   const int g = T == 64 ? 0 : lane / T;
   const int s0 = start[b], cnt = count[b];
@@ -1122,7 +1131,8 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
       const int2 q = rg[r];
       float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
       if (r < nrb.x)
-        field_range_mixed<G>(rec, q.x + g, q.y, tx, ty, tz, fc, a1x, a1y, a1z, a2x, a2y, a2z);
+        field_range_mixed<G>(rec, q.x + g, q.y, tx, ty, tz, fc, s_ftab, a1x, a1y, a1z, a2x, a2y,
+                             a2z);
       else
         field_range_far<G>(rec, q.x + g, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
       acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
@@ -1501,6 +1511,12 @@ struct FmmTables {
   int slot_ld = 0;
 };
 
+// Near-pair coefficient table of one smoothing length (make_field_table),
+// built once per context and h.
+struct FieldTable {
+  DeviceBuffer buf;  // kFieldTabN x 8 floats
+};
+
 struct FmmState {
   int P = 10, KP = 128;
   int D = 0;
@@ -1510,6 +1526,7 @@ struct FmmState {
   double L = 0.0, s_leaf = 0.0;
   int64_t n = 0;
   std::shared_ptr<FmmTables> tab;
+  std::shared_ptr<FieldTable> ftab;
   DeviceBuffer keys, keys_sorted, idx, perm, flags, scan, rec;  // particle arrays
   // boxes: level l occupies [base[l], base[l] + count[l]) of the global arrays
   std::vector<int> base, count;
@@ -1693,6 +1710,26 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
   return Status::Ok();
 }
 
+Status build_field_table(mf_ctx *ctx, double h, std::shared_ptr<FieldTable> *out) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  char key[64];
+  std::snprintf(key, sizeof(key), "field_table_h%a", h);
+  auto it = ctx->cache.find(key);
+  if (it != ctx->cache.end()) {
+    *out = std::static_pointer_cast<FieldTable>(it->second);
+    return Status::Ok();
+  }
+  auto t = std::make_shared<FieldTable>();
+  std::vector<float> tab(8 * kFieldTabN);
+  make_field_table(h, tab.data());
+  MF_TRY(t->buf.reserve(tab.size() * sizeof(float)));
+  MF_HIP(hipMemcpy(t->buf.get(), tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->cache[key] = t;
+  *out = t;
+  return Status::Ok();
+}
+
 template <class T>
 Status need(DeviceBuffer &b, int64_t count) {
   return b.reserve(sizeof(T) * static_cast<size_t>(count > 0 ? count : 1));
@@ -1717,6 +1754,7 @@ Status fmm_build(mf_plan *plan) {
     return Status::Err(MF_ERR_INVALID_ARG, "FMM expansion order must be 6, 8 or 10");
   F.KP = kp_for(F.P);
   MF_TRY(build_tables(ctx, F.P, &F.tab));
+  MF_TRY(build_field_table(ctx, plan->h, &F.ftab));
   const FmmTables &tab = *F.tab;
   const int64_t n = plan->n;
   F.n = n;
@@ -2321,8 +2359,8 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
                        0, ns, lists + F.leaf_list_off[k],                                      \
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
-                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc, h0, h1, h2,    \
-                       ox, oy, oz);                                                              \
+                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc,                \
+                       F.ftab->buf.as<float4>(), h0, h1, h2, ox, oy, oz);                        \
     MF_LAUNCH_CHECK();                                                                           \
   }
       MF_P2P_FIELD(64, 0)

@@ -96,6 +96,23 @@ def test_fmm_small_and_shallow_trees(ctx, oracle, shape):
         assert rel_err(F, oracle.force_allpairs(pos, m, 2.0)) < 5e-6
 
 
+def test_fmm_near_table_dense_cloud(ctx, oracle):
+    """1000 uniform random particles in a cube of side 7.9 (h = 2): the tree
+    has no far field, so every pair goes through the near-field kernels, and
+    the pair distances cover q in [0, 2) densely (the full-leaf shape reads
+    alpha, beta from the 64-interval cubic table there).  The table is within
+    1.5e-7 of the exact coefficients, below FP32 product noise; a wrong
+    interval or coefficient shows orders of magnitude above the bound.  The
+    moments are all alike, so the field does not cancel."""
+    import magforce
+    rng = np.random.default_rng(51)
+    pos = f32(rng.uniform(0.0, 7.9, size=(1000, 3)))
+    m = f32(np.tile([0.2, -0.1, 0.6], (pos.shape[0], 1)) + 0.05 * rng.normal(size=(1000, 3)))
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    assert plan.info()["depth"] < 2
+    assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < 1e-6
+
+
 def test_fmm_sparse_deep_tree(ctx, oracle):
     """Widely spread clusters: many levels with few boxes each."""
     import magforce
