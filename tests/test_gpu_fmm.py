@@ -113,6 +113,23 @@ def test_fmm_near_table_dense_cloud(ctx, oracle):
     assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < 1e-6
 
 
+def test_fmm_coincident_particles(ctx, oracle):
+    """Exact duplicates inside a slab with a far field (r = 0 pairs between
+    distinct particles): the field pair takes its q = 0 value (the self term)
+    and the force pair is zero (Dipole_B.kern:114-128,
+    Dipole_hessian.kern:101-104), in the FMM near field as in the oracle."""
+    import magforce
+    base = slab(24, 24, 6, seed=61)
+    rng = np.random.default_rng(62)
+    dup = base[rng.choice(base.shape[0], size=300, replace=False)]
+    pos = np.concatenate([base, dup])
+    m = f32(random_moments(pos.shape[0], seed=63, scale=0.4))
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM)
+    assert plan.info()["depth"] >= 2
+    assert rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, 2.0)) < TOL[10]
+    assert rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, 2.0)) < TOL[10]
+
+
 def test_fmm_sparse_deep_tree(ctx, oracle):
     """Widely spread clusters: many levels with few boxes each."""
     import magforce
