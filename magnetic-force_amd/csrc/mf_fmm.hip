@@ -232,7 +232,7 @@ __global__ void set_parent_kernel(int nchild, const int32_t *__restrict__ child_
 // Near-field lists of the leaves: list[b * nslot + k] = leaf index or -1.
 __global__ void near_list_kernel(int nleaf, const uint64_t *__restrict__ leaf_keys, int cells,
                                  int shift, const int3 *__restrict__ offs, int nslot,
-                                 int32_t *__restrict__ list) {
+                                 const int32_t *__restrict__ grid, int32_t *__restrict__ list) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<int64_t>(nleaf) * nslot) return;
   const int b = static_cast<int>(t / nslot), k = static_cast<int>(t % nslot);
@@ -243,15 +243,30 @@ __global__ void near_list_kernel(int nleaf, const uint64_t *__restrict__ leaf_ke
   const int x = c.x + o.x, y = c.y + o.y, z = c.z + o.z;
   int r = -1;
   if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells)
-    r = find_key(leaf_keys, nleaf, sys_bits | morton(x, y, z));
+    r = grid ? grid[morton(x, y, z)] : find_key(leaf_keys, nleaf, sys_bits | morton(x, y, z));
   list[t] = r;
+}
+
+// Dense cell -> box grid of one level of a single-system plan: the level's
+// keys are plain Morton codes of 3l bits, so grid[key] = box (level-local),
+// -1 for an empty cell.  The list kernels then look a neighbour up with one
+// load instead of a binary search over the level's keys.
+__global__ void grid_clear_kernel(int64_t n, int32_t *__restrict__ grid) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) grid[i] = -1;
+}
+
+__global__ void grid_scatter_kernel(int nbox, const uint64_t *__restrict__ keys,
+                                    int32_t *__restrict__ grid) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nbox) grid[keys[b]] = b;
 }
 
 // M2L source slots of the boxes of one level: src[b * nslot + k] = global
 // index of the source box at offset offs[parity][k] (target - source), or -1.
 __global__ void m2l_list_kernel(int nbox, int box_base, const uint64_t *__restrict__ keys,
                                 int cells, int shift, const int3 *__restrict__ offs, int nslot,
-                                int32_t *__restrict__ src) {
+                                const int32_t *__restrict__ grid, int32_t *__restrict__ src) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= static_cast<int64_t>(nbox) * nslot) return;
   const int b = static_cast<int>(t / nslot), k = static_cast<int>(t % nslot);
@@ -264,7 +279,7 @@ __global__ void m2l_list_kernel(int nbox, int box_base, const uint64_t *__restri
   if (o.x != 0x7fffffff) {
     const int x = c.x - o.x, y = c.y - o.y, z = c.z - o.z;
     if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells) {
-      const int f = find_key(keys, nbox, sys_bits | morton(x, y, z));
+      const int f = grid ? grid[morton(x, y, z)] : find_key(keys, nbox, sys_bits | morton(x, y, z));
       r = f >= 0 ? box_base + f : -1;
     }
   }
@@ -633,12 +648,26 @@ __global__ __launch_bounds__(256) void tile_slots_kernel(const M2LTile *__restri
       if (src_slots[static_cast<int64_t>(tgt) * src_ld + k] >= 0) s_flag[k] = 1;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    int c = 0;
-    for (int k = 0; k < tile.nslot; ++k)
-      if (s_flag[k] && smap[k] >= 0) list[static_cast<int64_t>(blockIdx.x) * list_ld + c++] = k;
-    nlist[blockIdx.x] = c;
+  // Compaction in slot order, 256 slots per pass: a slot's position is the
+  // count of flagged slots before it (earlier passes, earlier waves, lower
+  // lanes of its own wave).
+  __shared__ int s_wave[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t *__restrict__ out = list + static_cast<int64_t>(blockIdx.x) * list_ld;
+  int base = 0;
+  for (int k0 = 0; k0 < tile.nslot; k0 += 256) {
+    const int k = k0 + static_cast<int>(threadIdx.x);
+    const bool f = k < tile.nslot && s_flag[k] != 0 && smap[k] >= 0;
+    const uint64_t m = __ballot(f);
+    if (lane == 0) s_wave[w] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int i = 0; i < w; ++i) off += s_wave[i];
+    if (f) out[off + __popcll(m & ((1ULL << lane) - 1ULL))] = k;
+    base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();  // s_wave is rewritten by the next pass
   }
+  if (threadIdx.x == 0) nlist[blockIdx.x] = base;
 }
 
 // child_tab[parent * 8 + octant] = global index of the child (one level).
@@ -1540,6 +1569,7 @@ struct FmmState {
   DeviceBuffer box_count;    // int32 [nbox]: particles (leaf) / children (inner)
   DeviceBuffer box_parent;   // int32 [nbox] global parent index
   DeviceBuffer nearF, nearH; // leaf lists
+  DeviceBuffer cell_grid;    // int32 [8^l]: cell -> box of one level (list building)
   DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
   DeviceBuffer nrF, nrH;     // int2 range counts per leaf (class 0, class 1)
   DeviceBuffer leaf_lists;   // int32: the four leaf-shape lists back to back, each in leaf order
@@ -1924,17 +1954,35 @@ Status fmm_build(mf_plan *plan) {
   }
   F.count = cnt;
   F.nbox = cursor;
+  // Cell grid of level l (see grid_scatter_kernel) for single-system plans of
+  // at most 2^24 cells per level; nullptr (binary search) otherwise.
+  auto level_grid = [&](int l, const int32_t **out) -> Status {
+    *out = nullptr;
+    if (nsys != 1 || 3 * l > 24 || cnt[l] == 0) return Status::Ok();
+    const int64_t ncell = 1LL << (3 * l);
+    MF_TRY(need<int32_t>(F.cell_grid, ncell));
+    hipLaunchKernelGGL(grid_clear_kernel, dim3(grid_for(ncell, 256)), dim3(256), 0, ctx->stream,
+                       ncell, F.cell_grid.as<int32_t>());
+    MF_LAUNCH_CHECK();
+    hipLaunchKernelGGL(grid_scatter_kernel, dim3(grid_for(cnt[l], 256)), dim3(256), 0, ctx->stream,
+                       cnt[l], F.box_keys.as<uint64_t>() + F.base[l], F.cell_grid.as<int32_t>());
+    MF_LAUNCH_CHECK();
+    *out = F.cell_grid.as<int32_t>();
+    return Status::Ok();
+  };
   // 4. near lists of the leaves
   const int nleaf = cnt[D];
+  const int32_t *leaf_grid = nullptr;
+  MF_TRY(level_grid(D, &leaf_grid));
   MF_TRY(need<int32_t>(F.nearF, static_cast<int64_t>(nleaf) * kNearF));
   MF_TRY(need<int32_t>(F.nearH, static_cast<int64_t>(nleaf) * kNearH));
   hipLaunchKernelGGL(near_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * kNearF, 256)),
                      dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells, 3 * D,
-                     tab.nearF.as<int3>(), kNearF, F.nearF.as<int32_t>());
+                     tab.nearF.as<int3>(), kNearF, leaf_grid, F.nearF.as<int32_t>());
   MF_LAUNCH_CHECK();
   hipLaunchKernelGGL(near_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * kNearH, 256)),
                      dim3(256), 0, ctx->stream, nleaf, F.box_keys.as<uint64_t>(), cells, 3 * D,
-                     tab.nearH.as<int3>(), kNearH, F.nearH.as<int32_t>());
+                     tab.nearH.as<int3>(), kNearH, leaf_grid, F.nearH.as<int32_t>());
   MF_LAUNCH_CHECK();
   MF_TRY(need<int2>(F.rngF, static_cast<int64_t>(nleaf) * kNearF));
   MF_TRY(need<int2>(F.rngH, static_cast<int64_t>(nleaf) * kNearH));
@@ -1973,17 +2021,21 @@ Status fmm_build(mf_plan *plan) {
   MF_TRY(need<int32_t>(F.srcH, static_cast<int64_t>(nleaf) * tab.nslotH));
   for (int l = 2; l <= D; ++l) {
     if (cnt[l] == 0) continue;
+    const int32_t *g = nullptr;
+    MF_TRY(level_grid(l, &g));
     hipLaunchKernelGGL(m2l_list_kernel, dim3(grid_for(static_cast<int64_t>(cnt[l]) * tab.nslotF, 256)),
                        dim3(256), 0, ctx->stream, cnt[l], F.base[l],
                        F.box_keys.as<uint64_t>() + F.base[l], 1 << l, 3 * l, tab.offsF.as<int3>(),
-                       tab.nslotF, F.srcF.as<int32_t>());
+                       tab.nslotF, g, F.srcF.as<int32_t>());
     MF_LAUNCH_CHECK();
   }
   if (D >= 2) {
     // leaf level Hessian slots: indexed by leaf (global index == leaf index, base[D] = 0)
+    const int32_t *g = nullptr;
+    MF_TRY(level_grid(D, &g));
     hipLaunchKernelGGL(m2l_list_kernel, dim3(grid_for(static_cast<int64_t>(nleaf) * tab.nslotH, 256)),
                        dim3(256), 0, ctx->stream, nleaf, 0, F.box_keys.as<uint64_t>(), cells, 3 * D,
-                       tab.offsH.as<int3>(), tab.nslotH, F.srcH.as<int32_t>());
+                       tab.offsH.as<int3>(), tab.nslotH, g, F.srcH.as<int32_t>());
     MF_LAUNCH_CHECK();
   }
   int nm2l = 0;
