@@ -369,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void force_allpairs_kernel(
           const float ir2 = ir * ir;
           float P = ir2 * ir2 * ir;
           float R = -5.f * P * ir2;
-          if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.curve_scale, hc.inv_h, r2, ir, P, R);
+          if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, P, R);
           const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
           const float dn = fmaf(dx, nx[k], fmaf(dy, ny[k], dz * nz[k]));
           const float mn = fmaf(mx, nx[k], fmaf(my, ny[k], mz * nz[k]));
@@ -439,7 +439,7 @@ __global__ __launch_bounds__(kBlock) void hessian_allpairs_kernel(
       const float ir2 = ir * ir;
       float P = ir2 * ir2 * ir;
       float R = -5.f * P * ir2;
-      if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.curve_scale, hc.inv_h, r2, ir, P, R);
+      if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, P, R);
       const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
       const float Rdm = R * dm;
       const float Pdm = P * dm;
@@ -645,7 +645,7 @@ HessConsts make_hess_consts(double h) {
   hc.inv_h = static_cast<float>(1.0 / h);
   hc.near2 = static_cast<float>(16.0 * h * h);
   const double s = 3.0 * kMu0 / (4.0 * kPi);
-  hc.curve_scale = static_cast<float>(1.0 / (s * h * h * h * h));
+  const double scale = 1.0 / (s * h * h * h * h);
   for (int k = 0; k < 4; ++k) {
     double d[5];
     for (int j = 0; j < 5; ++j) d[j] = kC2[k][j] - 3.0 * kC1[k][j];
@@ -653,8 +653,8 @@ HessConsts make_hess_consts(double h) {
     shift_quartic(kC1[k], k + 0.5, c1s);
     shift_quartic(d, k + 0.5, ds);
     for (int j = 0; j < 5; ++j) {
-      hc.curve[k][j] = static_cast<float>(c1s[j]);
-      hc.curve[k][5 + j] = static_cast<float>(ds[j]);
+      hc.curve[k][j] = static_cast<float>(scale * c1s[j]);
+      hc.curve[k][5 + j] = static_cast<float>(scale * ds[j]);
     }
     hc.curve[k][10] = hc.curve[k][11] = 0.f;
   }

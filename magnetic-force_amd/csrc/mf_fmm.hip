@@ -291,11 +291,19 @@ __global__ void class_key_kernel(int nbox, int box_base, int level,
                                  const uint64_t *__restrict__ keys, uint64_t *__restrict__ ckey,
                                  int32_t *__restrict__ ids, int32_t *__restrict__ counts) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbox) return;
-  const int parity = static_cast<int>(keys[b] & 7ULL);
-  ckey[b] = (static_cast<uint64_t>(level) << 3) | static_cast<uint64_t>(parity);
-  ids[b] = box_base + b;
-  atomicAdd(&counts[level * 8 + parity], 1);
+  const int parity = b < nbox ? static_cast<int>(keys[b] & 7ULL) : -1;
+  if (b < nbox) {
+    ckey[b] = (static_cast<uint64_t>(level) << 3) | static_cast<uint64_t>(parity);
+    ids[b] = box_base + b;
+  }
+  // one counter update per parity present in the wave
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    const uint64_t m = __ballot(parity == p);
+    if (m != 0 && lane == __ffsll(static_cast<long long>(m)) - 1)
+      atomicAdd(&counts[level * 8 + p], __popcll(m));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1334,7 +1342,7 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
         const float ir2 = ir * ir;
         float Pc = ir2 * ir2 * ir;
         float Rc = -5.f * Pc * ir2;
-        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.curve_scale, hc.inv_h, r2, ir, Pc, Rc);
+        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
         const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
         const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
         const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
@@ -1388,7 +1396,7 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
         const float ir2 = ir * ir;
         float Pc = ir2 * ir2 * ir;
         float Rc = -5.f * Pc * ir2;
-        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.curve_scale, hc.inv_h, r2, ir, Pc, Rc);
+        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
         const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
         const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
         const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));

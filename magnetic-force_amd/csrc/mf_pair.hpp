@@ -63,9 +63,9 @@ struct HessConsts {
   float h;
   float inv_h;
   float near2;       // (4h)^2
-  float curve_scale; // 1/(s h^4), s = 3 mu0/(4 pi)
   // curve[k][0..4] = C1 re-centred at k+0.5 (ascending powers),
-  // curve[k][5..9] = C2 - 3 C1 re-centred likewise; [10..11] padding.
+  // curve[k][5..9] = C2 - 3 C1 re-centred likewise, both times 1/(s h^4),
+  // s = 3 mu0/(4 pi), in FP64 before rounding; [10..11] padding.
   float curve[4][12];
 };
 
@@ -120,9 +120,8 @@ __device__ __forceinline__ void field_near_coeffs_tab(const float4 *tab, float t
 
 // Force pair coefficients for the near field (q <= 4).  `tab` points at the
 // 4 x 12 curve table in LDS.  P, R as in the header comment (scaled by 1/s).
-__device__ __forceinline__ void hess_near_coeffs(const float *tab, float curve_scale,
-                                                 float inv_h, float r2, float ir, float &P,
-                                                 float &R) {
+__device__ __forceinline__ void hess_near_coeffs(const float *tab, float inv_h, float r2,
+                                                 float ir, float &P, float &R) {
   const float r = r2 * ir;
   const float q = r * inv_h;
   int k = static_cast<int>(ceilf(q)) - 1;
@@ -134,8 +133,8 @@ __device__ __forceinline__ void hess_near_coeffs(const float *tab, float curve_s
   float c1 = fmaf(fmaf(fmaf(fmaf(v1.x, t, v0.w), t, v0.z), t, v0.y), t, v0.x);
   float dd = fmaf(fmaf(fmaf(fmaf(v2.y, t, v2.x), t, v1.w), t, v1.z), t, v1.y);
   const bool live = r2 > 0.f;
-  P = live ? c1 * curve_scale * ir : 0.f;
-  R = live ? dd * curve_scale * (ir * ir * ir) : 0.f;
+  P = live ? c1 * ir : 0.f;
+  R = live ? dd * (ir * ir * ir) : 0.f;
 }
 
 }  // namespace mf
