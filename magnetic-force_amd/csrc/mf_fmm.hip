@@ -989,7 +989,7 @@ __device__ __forceinline__ double group_sum(double v) {
 
 // Near-source stream of a sparse leaf (the T < 64 shapes).  The leaf's near
 // ranges, concatenated, are walked in windows of kStreamW records: lane l
-// loads records w0 + l + 64 r (coalesced loads, issued while the previous
+// loads stream positions w0 + kStreamR l + r (issued while the previous
 // window is evaluated), the window goes through LDS, and lane group g takes
 // records g, g + G, ... of it.  A range-by-range walk instead waits once per
 // G sources on a load, which dominated these leaves' time.
@@ -1028,26 +1028,28 @@ __device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__re
   return carry;
 }
 
-// Record of stream position i < total: the last range r with pre[r] <= i
-// (empty ranges share their successor's offset and are skipped by "last").
-__device__ __forceinline__ int near_stream_record(const NearStream &S, int nrange, int i) {
-  int lo = 0, hi = nrange - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (S.pre[mid] <= i) lo = mid; else hi = mid - 1;
-  }
-  return S.beg[lo] + (i - S.pre[lo]);
-}
-
-// Loads the window starting at w0 into (a, b) (lane l: positions w0 + l + 64 r).
+// Loads the window starting at w0 into (a, b).  Position i < total is record
+// beg[r] + i - pre[r] of the last range r with pre[r] <= i (empty ranges share
+// their successor's offset and are skipped by "last").  Lane l takes the
+// kStreamR consecutive positions w0 + kStreamR l + r: one binary search finds
+// the range of the first, and the others step forward from it
+// (pre[nrange] = total ends the step).
 __device__ __forceinline__ void near_stream_fetch(const NearStream &S, const SrcRec *__restrict__ rec,
                                                   int nrange, int total, int w0, int lane,
                                                   float4 (&a)[kStreamR], float4 (&b)[kStreamR]) {
+  const int i0 = w0 + kStreamR * lane;
+  if (i0 >= total) return;
+  int lo = 0, hi = nrange - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (S.pre[mid] <= i0) lo = mid; else hi = mid - 1;
+  }
 #pragma unroll
   for (int r = 0; r < kStreamR; ++r) {
-    const int i = w0 + lane + 64 * r;
+    const int i = i0 + r;
     if (i < total) {
-      const float4 *p = reinterpret_cast<const float4 *>(rec + near_stream_record(S, nrange, i));
+      while (S.pre[lo + 1] <= i) ++lo;
+      const float4 *p = reinterpret_cast<const float4 *>(rec + S.beg[lo] + (i - S.pre[lo]));
       a[r] = p[0];
       b[r] = p[1];
     }
@@ -1060,8 +1062,8 @@ __device__ __forceinline__ void near_stream_publish(NearStream &S, int lane, con
   __syncthreads();  // the previous window is consumed
 #pragma unroll
   for (int r = 0; r < kStreamR; ++r) {
-    S.win[0][lane + 64 * r] = a[r];
-    S.win[1][lane + 64 * r] = b[r];
+    S.win[0][kStreamR * lane + r] = a[r];
+    S.win[1][kStreamR * lane + r] = b[r];
   }
   __syncthreads();
 }
