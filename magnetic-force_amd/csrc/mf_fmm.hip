@@ -1128,6 +1128,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
       if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
       const int n = min(kStreamW, total - w0);
       float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
+#pragma unroll 2
       for (int k = g; k < n; k += G) {
         const float4 pa = S.win[0][k], pb = S.win[1][k];
         const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
@@ -1335,6 +1336,7 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
       if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
       const int n = min(kStreamW, total - w0);
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
+#pragma unroll 2
       for (int k = g; k < n; k += G) {
         const float4 pa = S.win[0][k], pb = S.win[1][k];
         const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
