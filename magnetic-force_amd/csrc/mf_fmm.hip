@@ -988,22 +988,26 @@ __device__ __forceinline__ double group_sum(double v) {
 }
 
 // Near-source stream of a sparse leaf (the T < 64 shapes).  The leaf's near
-// ranges, concatenated, are walked in windows of kStreamW records: lane l
-// loads stream positions w0 + kStreamR l + r (issued while the previous
-// window is evaluated), the window goes through LDS, and lane group g takes
-// records g, g + G, ... of it.  A range-by-range walk instead waits once per
-// G sources on a load, which dominated these leaves' time.
+// ranges, concatenated, are walked in windows of kStreamW records by the
+// kStreamWaves waves of the workgroup, wave w taking windows w, w + W, ...:
+// lane l loads stream positions w0 + kStreamR l + r (issued while the
+// previous window is evaluated), the window goes through the wave's LDS
+// buffer, and lane group g takes records g, g + G, ... of it.  A
+// range-by-range walk instead waits once per G sources on a load, and one
+// wave per leaf left a few leaves with long streams setting the launch time.
 constexpr int kStreamR = 2;              // records per lane per window
 constexpr int kStreamW = 64 * kStreamR;  // window length
+constexpr int kStreamWaves = 4;          // waves per sparse leaf
 
 struct NearStream {
-  int pre[kNearH + 1];      // stream offset of each range; pre[nrange] = total
-  int beg[kNearH];          // first record of each range
-  float4 win[2][kStreamW];  // the window: record halves {x, y, z, -}, {mx, my, mz, -}
+  int pre[kNearH + 1];  // stream offset of each range; pre[nrange] = total
+  int beg[kNearH];      // first record of each range
+  // per wave, the window: record halves {x, y, z, -}, {mx, my, mz, -}
+  float4 win[kStreamWaves][2][kStreamW];
 };
 
-// Fills pre/beg from the leaf's ranges (one wave); returns the total.
-__device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__restrict__ rg,
+// Fills pre/beg from the leaf's ranges (one wave).
+__device__ __forceinline__ void near_stream_scan(NearStream &S, const int2 *__restrict__ rg,
                                                  int nrange, int lane) {
   int carry = 0;
   for (int c = 0; c < nrange; c += 64) {
@@ -1024,8 +1028,14 @@ __device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__re
     carry += __shfl(incl, 63, 64);
   }
   if (lane == 0) S.pre[nrange] = carry;
+}
+
+// The scan by wave 0, then a workgroup barrier; returns the total.
+__device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__restrict__ rg,
+                                                 int nrange, int lane, int wave) {
+  if (wave == 0) near_stream_scan(S, rg, nrange, lane);
   __syncthreads();
-  return carry;
+  return S.pre[nrange];
 }
 
 // Loads the window starting at w0 into (a, b).  Position i < total is record
@@ -1056,33 +1066,37 @@ __device__ __forceinline__ void near_stream_fetch(const NearStream &S, const Src
   }
 }
 
-// Publishes the fetched window in LDS (after the previous one is consumed).
-__device__ __forceinline__ void near_stream_publish(NearStream &S, int lane, const float4 (&a)[kStreamR],
+// Publishes the fetched window in the wave's LDS buffer (after the previous
+// one is consumed).  Every wave of the workgroup calls it the same number of
+// times.
+__device__ __forceinline__ void near_stream_publish(NearStream &S, int wave, int lane,
+                                                    const float4 (&a)[kStreamR],
                                                     const float4 (&b)[kStreamR]) {
   __syncthreads();  // the previous window is consumed
 #pragma unroll
   for (int r = 0; r < kStreamR; ++r) {
-    S.win[0][kStreamR * lane + r] = a[r];
-    S.win[1][kStreamR * lane + r] = b[r];
+    S.win[wave][0][kStreamR * lane + r] = a[r];
+    S.win[wave][1][kStreamR * lane + r] = b[r];
   }
   __syncthreads();
 }
 
 // P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
-// writes H_const + near (trees without a far field).  One wave per leaf, in
-// one of two shapes:
-//  * T = 64: one target per lane, every lane walks all sources (scalar
-//    loads), over the leaf's full chunks of 64 targets -- and over the
-//    remainder too when it holds more than 32;
+// writes H_const + near (trees without a far field).  One workgroup per
+// leaf, in one of two shapes:
+//  * T = 64: one wave; one target per lane, every lane walks all sources
+//    (scalar loads), over the leaf's full chunks of 64 targets -- and over
+//    the remainder too when it holds more than 32;
 //  * T = 32, 16, 8, for a remainder of at most T targets (a sparse leaf, or
-//    the tail of a leaf of 65..79 particles): the wave is split into
+//    the tail of a leaf of 65..79 particles): kStreamWaves waves share the
+//    leaf's source stream window by window, and each wave is split into
 //    G = 64 / T groups of T lanes; every group holds all remainder targets
-//    and takes every G-th source of each range (vector loads), and the
-//    groups' partial sums are added across lanes.  A short remainder then
-//    keeps (most of) the wave busy instead of idling 64 - r lanes.
+//    and takes every G-th source of a window, the groups' partial sums are
+//    added across lanes and the waves' in wave order.  A short remainder
+//    then keeps (most of) the waves busy instead of idling 64 - r lanes.
 // leaves: the leaf indices this launch covers (blockIdx.x -> leaf).
 template <int T>
-__global__ __launch_bounds__(64) void p2p_field_kernel(
+__global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_field_kernel(
     const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
@@ -1092,7 +1106,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
   const int li = xcd_contiguous(nleaves);
   if (li < 0) return;
   const int b = leaves[li];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   // The full-leaf shape takes its near-pair coefficients from the table
   // (2 KB of LDS).  The sparse shapes keep the closed form: their waves wait
   // on source windows, and the extra LDS would cost them occupancy.
@@ -1110,10 +1124,13 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double cf = 1.0 / (4.0 * kPi);
   if constexpr (T < 64) {
-    // at most T targets: one pass over the near-source stream
+    // at most T targets: one pass over the near-source stream, its windows
+    // shared out over the workgroup's waves
     __shared__ NearStream S;
+    __shared__ double s_red[kStreamWaves - 1][T][3];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
     const int nrange = nrb.x + nrb.y;
-    const int total = near_stream_setup(S, rg, nrange, lane);
+    const int total = near_stream_setup(S, rg, nrange, lane, wave);
     if (tb >= te) return;  // uniform, after the setup barrier
     const int t = tb + tl;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
@@ -1122,15 +1139,21 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
     float4 wa[kStreamR], wb[kStreamR];
 #pragma unroll
     for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-    near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
-    for (int w0 = 0; w0 < total; w0 += kStreamW) {
-      near_stream_publish(S, lane, wa, wb);
-      if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
-      const int n = min(kStreamW, total - w0);
+    const int nwin = (total + kStreamW - 1) / kStreamW;
+    const int rounds = (nwin + kStreamWaves - 1) / kStreamWaves;  // uniform
+    near_stream_fetch(S, rec, nrange, total, wave * kStreamW, lane, wa, wb);
+    for (int rd = 0; rd < rounds; ++rd) {
+      const int w0 = (rd * kStreamWaves + wave) * kStreamW;
+      near_stream_publish(S, wave, lane, wa, wb);
+      const int wn = w0 + kStreamWaves * kStreamW;
+      if (wn < total) near_stream_fetch(S, rec, nrange, total, wn, lane, wa, wb);
+      const int n = w0 < total ? min(kStreamW, total - w0) : 0;
+      const float4 *__restrict__ wx = S.win[wave][0];
+      const float4 *__restrict__ wm = S.win[wave][1];
       float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
 #pragma unroll 2
       for (int k = g; k < n; k += G) {
-        const float4 pa = S.win[0][k], pb = S.win[1][k];
+        const float4 pa = wx[k], pb = wm[k];
         const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
         const float mx = pb.x, my = pb.y, mz = pb.z;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1155,7 +1178,20 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
     acc0 = group_sum<T>(acc0);
     acc1 = group_sum<T>(acc1);
     acc2 = group_sum<T>(acc2);
-    if (g == 0 && t < te) {
+    // the waves' partial sums, added in wave order
+    if (wave > 0 && g == 0) {
+      s_red[wave - 1][tl][0] = acc0;
+      s_red[wave - 1][tl][1] = acc1;
+      s_red[wave - 1][tl][2] = acc2;
+    }
+    __syncthreads();
+    if (wave == 0 && g == 0 && t < te) {
+#pragma unroll
+      for (int w = 0; w < kStreamWaves - 1; ++w) {
+        acc0 += s_red[w][tl][0];
+        acc1 += s_red[w][tl][1];
+        acc2 += s_red[w][tl][2];
+      }
       const int o = perm[s0 + t];
       const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
       const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
@@ -1292,7 +1328,7 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
 // P2P, force: F[perm] (+)= (sum over the leaf's near ranges of B[m_j]) . m_i.
 // Leaf shapes (T) as in p2p_field_kernel.
 template <int T>
-__global__ __launch_bounds__(64) void p2p_force_kernel(
+__global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_kernel(
     const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
@@ -1309,7 +1345,7 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   __syncthreads();
   const float *tab = reinterpret_cast<const float *>(s_tab);
   const int b = leaves[li];
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int tl = T == 64 ? lane : lane % T;
   const int g = T == 64 ? 0 : lane / T;
   const int s0 = start[b], cnt = count[b];
@@ -1319,9 +1355,12 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double sfac = 3.0 * kMu0 / (4.0 * kPi);
   if constexpr (T < 64) {
-    // at most T targets: one pass over the near-source stream
+    // at most T targets: one pass over the near-source stream, its windows
+    // shared out over the workgroup's waves
     __shared__ NearStream S;
-    const int total = near_stream_setup(S, rg, nrange, lane);
+    __shared__ double s_red[kStreamWaves - 1][T][3];
+    const int wave = static_cast<int>(threadIdx.x) >> 6;
+    const int total = near_stream_setup(S, rg, nrange, lane, wave);
     if (tb >= te) return;  // uniform, after the setup barrier
     const int t = tb + tl;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
@@ -1330,15 +1369,21 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     float4 wa[kStreamR], wb[kStreamR];
 #pragma unroll
     for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-    near_stream_fetch(S, rec, nrange, total, 0, lane, wa, wb);
-    for (int w0 = 0; w0 < total; w0 += kStreamW) {
-      near_stream_publish(S, lane, wa, wb);
-      if (w0 + kStreamW < total) near_stream_fetch(S, rec, nrange, total, w0 + kStreamW, lane, wa, wb);
-      const int n = min(kStreamW, total - w0);
+    const int nwin = (total + kStreamW - 1) / kStreamW;
+    const int rounds = (nwin + kStreamWaves - 1) / kStreamWaves;  // uniform
+    near_stream_fetch(S, rec, nrange, total, wave * kStreamW, lane, wa, wb);
+    for (int rd = 0; rd < rounds; ++rd) {
+      const int w0 = (rd * kStreamWaves + wave) * kStreamW;
+      near_stream_publish(S, wave, lane, wa, wb);
+      const int wn = w0 + kStreamWaves * kStreamW;
+      if (wn < total) near_stream_fetch(S, rec, nrange, total, wn, lane, wa, wb);
+      const int n = w0 < total ? min(kStreamW, total - w0) : 0;
+      const float4 *__restrict__ wx = S.win[wave][0];
+      const float4 *__restrict__ wm = S.win[wave][1];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
 #pragma unroll 2
       for (int k = g; k < n; k += G) {
-        const float4 pa = S.win[0][k], pb = S.win[1][k];
+        const float4 pa = wx[k], pb = wm[k];
         const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
         const float mx = pb.x, my = pb.y, mz = pb.z;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -1374,7 +1419,20 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     f0 = group_sum<T>(f0);
     f1 = group_sum<T>(f1);
     f2 = group_sum<T>(f2);
-    if (g == 0 && t < te) {
+    // the waves' partial sums, added in wave order
+    if (wave > 0 && g == 0) {
+      s_red[wave - 1][tl][0] = f0;
+      s_red[wave - 1][tl][1] = f1;
+      s_red[wave - 1][tl][2] = f2;
+    }
+    __syncthreads();
+    if (wave == 0 && g == 0 && t < te) {
+#pragma unroll
+      for (int w = 0; w < kStreamWaves - 1; ++w) {
+        f0 += s_red[w][tl][0];
+        f1 += s_red[w][tl][1];
+        f2 += s_red[w][tl][2];
+      }
       const int o = perm[s0 + t];
       fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
       fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
@@ -2423,7 +2481,8 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FIELD(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)),          \
+                       dim3(T == 64 ? 64 : 64 * kStreamWaves),                                  \
                        0, ns, lists + F.leaf_list_off[k],                                      \
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
@@ -2480,7 +2539,8 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       const int32_t *lists = F.leaf_lists.as<int32_t>();
 #define MF_P2P_FORCE(T, k)                                                                       \
   if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)), dim3(64), \
+    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)),          \
+                       dim3(T == 64 ? 64 : 64 * kStreamWaves),                                  \
                        0, ns, lists + F.leaf_list_off[k],                                      \
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
