@@ -998,6 +998,7 @@ __device__ __forceinline__ double group_sum(double v) {
 constexpr int kStreamR = 2;              // records per lane per window
 constexpr int kStreamW = 64 * kStreamR;  // window length
 constexpr int kStreamWaves = 4;          // waves per sparse leaf
+constexpr int kLeafLists = 7;            // leaf shapes T = 64, 32, 16, 8, 4, 2, 1
 
 struct NearStream {
   int pre[kNearH + 1];  // stream offset of each range; pre[nrange] = total
@@ -1087,8 +1088,8 @@ __device__ __forceinline__ void near_stream_publish(NearStream &S, int wave, int
 //  * T = 64: one wave; one target per lane, every lane walks all sources
 //    (scalar loads), over the leaf's full chunks of 64 targets -- and over
 //    the remainder too when it holds more than 32;
-//  * T = 32, 16, 8, for a remainder of at most T targets (a sparse leaf, or
-//    the tail of a leaf of 65..79 particles): kStreamWaves waves share the
+//  * T = 32, 16, 8, 4, 2, 1, for a remainder of T/2 < r <= T targets (a
+//    sparse leaf, or the tail of a leaf of 65..79 particles): kStreamWaves waves share the
 //    leaf's source stream window by window, and each wave is split into
 //    G = 64 / T groups of T lanes; every group holds all remainder targets
 //    and takes every G-th source of a window, the groups' partial sums are
@@ -1237,11 +1238,12 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_field_ke
 }
 
 // Leaf shapes for the near-field kernels.  List 0 (T = 64) holds every leaf
-// with a full chunk of 64 or a remainder of more than 32; lists 1..3 hold the
-// leaves whose remainder (cnt mod 64) is 17..32, 9..16 or 1..8.  A leaf can
-// be in list 0 and in one of the others: they cover disjoint targets.  Two
-// (list, leaf) entries per leaf (list 4 = none); a stable sort by list then
-// gives every list in leaf (Morton) order.
+// with a full chunk of 64 or a remainder of more than 32; lists 1..6 hold the
+// leaves whose remainder (cnt mod 64) is 17..32, 9..16, 5..8, 3..4, 2 or 1
+// (T = 32, 16, 8, 4, 2, 1: a remainder fills at least half of each lane
+// group).  A leaf can be in list 0 and in one of the others: they cover
+// disjoint targets.  Two (list, leaf) entries per leaf (list kLeafLists =
+// none); a stable sort by list then gives every list in leaf (Morton) order.
 __global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ count,
                                        uint64_t *__restrict__ key, int32_t *__restrict__ leaf,
                                        int32_t *__restrict__ sizes) {
@@ -1249,15 +1251,16 @@ __global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ co
   if (b >= nleaf) return;
   const int c = count[b];
   const int r = c & 63;
-  const int k0 = (c >= 64 || r > 32) ? 0 : 4;
-  const int k1 = (r > 0 && r <= 32) ? (r > 16 ? 1 : (r > 8 ? 2 : 3)) : 4;
+  const int k0 = (c >= 64 || r > 32) ? 0 : kLeafLists;
+  const int k1 = (r == 0 || r > 32) ? kLeafLists
+                 : r > 16 ? 1 : r > 8 ? 2 : r > 4 ? 3 : r > 2 ? 4 : r > 1 ? 5 : 6;
   key[2 * b] = static_cast<uint64_t>(k0);
   key[2 * b + 1] = static_cast<uint64_t>(k1);
   leaf[2 * b] = b;
   leaf[2 * b + 1] = b;
   // one atomic per wave and list instead of one per leaf
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < kLeafLists; ++k) {
     const uint64_t m = __ballot((k == 0 && k0 == 0) || k1 == k);
     if (m != 0 && (threadIdx.x & 63) == __ffsll(static_cast<long long>(m)) - 1)
       atomicAdd(&sizes[k], __popcll(m));
@@ -1643,8 +1646,8 @@ struct FmmState {
   DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
   DeviceBuffer nrF, nrH;     // int2 range counts per leaf (class 0, class 1)
   DeviceBuffer leaf_lists;   // int32: the four leaf-shape lists back to back, each in leaf order
-  int leaf_list_n[4] = {0, 0, 0, 0};
-  int leaf_list_off[4] = {0, 0, 0, 0};
+  int leaf_list_n[kLeafLists] = {};
+  int leaf_list_off[kLeafLists] = {};
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
@@ -2072,19 +2075,19 @@ Status fmm_build(mf_plan *plan) {
     MF_TRY(need<int32_t>(F.leaf_lists, ne));
     void *pk = nullptr, *pz = nullptr;
     MF_TRY(workspace(ctx, "fmm_leaf_keys", (sizeof(uint64_t) * 2 + sizeof(int32_t)) * ne, &pk));
-    MF_TRY(workspace(ctx, "fmm_leaf_sizes", sizeof(int32_t) * 4, &pz));
+    MF_TRY(workspace(ctx, "fmm_leaf_sizes", sizeof(int32_t) * kLeafLists, &pz));
     uint64_t *k_in = static_cast<uint64_t *>(pk), *k_out = k_in + ne;
     int32_t *v_in = reinterpret_cast<int32_t *>(k_out + ne);
-    MF_HIP(hipMemsetAsync(pz, 0, sizeof(int32_t) * 4, ctx->stream));
+    MF_HIP(hipMemsetAsync(pz, 0, sizeof(int32_t) * kLeafLists, ctx->stream));
     hipLaunchKernelGGL(classify_leaves_kernel, dim3(grid_for(nleaf, 256)), dim3(256), 0,
                        ctx->stream, nleaf, F.box_count.as<int32_t>(), k_in, v_in,
                        static_cast<int32_t *>(pz));
     MF_LAUNCH_CHECK();
     MF_TRY(sort_pairs_u64_i32(ctx, k_in, k_out, v_in, F.leaf_lists.as<int32_t>(), ne, 0, 3));
-    MF_HIP(hipMemcpyAsync(F.leaf_list_n, pz, sizeof(int32_t) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    MF_HIP(hipMemcpyAsync(F.leaf_list_n, pz, sizeof(int32_t) * kLeafLists, hipMemcpyDeviceToHost, ctx->stream));
     MF_HIP(hipStreamSynchronize(ctx->stream));
     F.leaf_list_off[0] = 0;
-    for (int k = 1; k < 4; ++k) F.leaf_list_off[k] = F.leaf_list_off[k - 1] + F.leaf_list_n[k - 1];
+    for (int k = 1; k < kLeafLists; ++k) F.leaf_list_off[k] = F.leaf_list_off[k - 1] + F.leaf_list_n[k - 1];
   }
   // 5. M2L source slots (levels >= 2) and the class-sorted tile lists
   MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox) * tab.nslotF));
@@ -2494,6 +2497,9 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       MF_P2P_FIELD(32, 1)
       MF_P2P_FIELD(16, 2)
       MF_P2P_FIELD(8, 3)
+      MF_P2P_FIELD(4, 4)
+      MF_P2P_FIELD(2, 5)
+      MF_P2P_FIELD(1, 6)
 #undef MF_P2P_FIELD
       timer.stop();
       return serial ? Status::Ok() : ov.record();
@@ -2551,6 +2557,9 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       MF_P2P_FORCE(32, 1)
       MF_P2P_FORCE(16, 2)
       MF_P2P_FORCE(8, 3)
+      MF_P2P_FORCE(4, 4)
+      MF_P2P_FORCE(2, 5)
+      MF_P2P_FORCE(1, 6)
 #undef MF_P2P_FORCE
       timer.stop();
       return serial ? Status::Ok() : ov.record();
