@@ -318,7 +318,11 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
                                                  const SrcRec *__restrict__ rec, Frame fr,
                                                  double s, float *__restrict__ Mt) {
   constexpr int ND = (P + 1) * (P + 1);
-  __shared__ float red[64][ND + 1];
+  // The wave's sums go through LDS in two halves of the dofs: a 64 x ND
+  // buffer (31 KB at p = 10) would hold P2M to 5 waves per CU.
+  constexpr int NH = (ND + 1) / 2;
+  static_assert(NH <= 64, "one dof per lane in each half");
+  __shared__ float red[64][NH + 1];
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   double cx, cy, cz;
@@ -349,14 +353,20 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
     }
   }
 #pragma unroll
-  for (int d = 0; d < ND; ++d) red[lane][d] = acc[d];
-  __syncthreads();
-  for (int d = lane; d < KP; d += 64) {
-    float sum = 0.f;
-    if (d < ND)
-      for (int l = 0; l < 64; ++l) sum += red[l][d];
-    Mt[static_cast<int64_t>(b) * KP + d] = sum * inv_s;
+  for (int h = 0; h < 2; ++h) {
+    if (h > 0) __syncthreads();  // the first half is read
+#pragma unroll
+    for (int d = 0; d < NH; ++d)
+      if (h * NH + d < ND) red[lane][d] = acc[h * NH + d];
+    __syncthreads();
+    const int d = h * NH + lane;
+    if (lane < NH && d < ND) {
+      float sum = 0.f;
+      for (int l = 0; l < 64; ++l) sum += red[l][lane];
+      Mt[static_cast<int64_t>(b) * KP + d] = sum * inv_s;
+    }
   }
+  for (int d = ND + lane; d < KP; d += 64) Mt[static_cast<int64_t>(b) * KP + d] = 0.f;
 }
 
 // ---------------------------------------------------------------------------
