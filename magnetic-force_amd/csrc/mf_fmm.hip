@@ -1619,6 +1619,8 @@ struct FmmTables {
   DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP x KP in fragment order]
   DeviceBuffer W_m2m;     // [8][KP x KP in FP32 fragment order]
   DeviceBuffer W_l2l;     // [8][KP x KP in FP32 fragment order]
+  DeviceBuffer W_m2m_b;   // BF16 bits [8][3 parts][KP x KP in fragment order]
+  DeviceBuffer W_l2l_b;   // BF16 bits [8][3 parts][KP x KP in fragment order]
   DeviceBuffer slot_tab;  // [kSlotRows][slot_ld] table index per slot (-1 unused)
   DeviceBuffer offsF;     // [8][nslotF] int3 (padding 0x7fffffff)
   DeviceBuffer offsH;     // [8][nslotH]
@@ -1770,6 +1772,22 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
       hoh[b * t->nslotH + k] = oh[b][k];
     }
   }
+  // BF16 parts of a KP x KP matrix (column-major FP64) at out[base ..], in
+  // fragment order: part p is the BF16 of what parts 0..p-1 left over.
+  auto split_bf16_parts = [KP](const std::vector<double> &m, uint16_t *out) {
+    for (int row = 0; row < KP; ++row)
+      for (int col = 0; col < KP; ++col) {
+        double r = m[static_cast<size_t>(col) * KP + row];
+        for (int p = 0; p < 3; ++p) {
+          const uint16_t bits = bf16_bits_rne_host(static_cast<float>(r));
+          out[bf16_frag_index(KP, p, row, col)] = bits;
+          uint32_t w = static_cast<uint32_t>(bits) << 16;
+          float f;
+          std::memcpy(&f, &w, sizeof(f));
+          r -= static_cast<double>(f);
+        }
+      }
+  };
   // M2L tables: FP32 and BF16 parts, both in fragment order
   std::vector<float> T(static_cast<size_t>(uniq.size()) * KP * KP);
   std::vector<uint16_t> Tb(static_cast<size_t>(uniq.size()) * 3 * KP * KP);
@@ -1778,20 +1796,10 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
     for (int col = 0; col < KP; ++col)
       for (int row = 0; row < KP; ++row)
         T[u * KP * KP + fp32_frag_index(KP, row, col)] = static_cast<float>(m[static_cast<size_t>(col) * KP + row]);
-    for (int row = 0; row < KP; ++row)
-      for (int col = 0; col < KP; ++col) {
-        double r = m[static_cast<size_t>(col) * KP + row];
-        for (int p = 0; p < 3; ++p) {
-          const uint16_t bits = bf16_bits_rne_host(static_cast<float>(r));
-          Tb[u * 3 * KP * KP + bf16_frag_index(KP, p, row, col)] = bits;
-          uint32_t w = static_cast<uint32_t>(bits) << 16;
-          float f;
-          std::memcpy(&f, &w, sizeof(f));
-          r -= static_cast<double>(f);
-        }
-      }
+    split_bf16_parts(m, Tb.data() + u * 3 * KP * KP);
   }
   std::vector<float> W1(8 * KP * KP), W2(8 * KP * KP);
+  std::vector<uint16_t> W1b(8 * 3 * KP * KP), W2b(8 * 3 * KP * KP);
   for (int oct = 0; oct < 8; ++oct) {
     // child octant bits (x, y, z) = (oct & 1, oct >> 1 & 1, oct >> 2 & 1); d in parent units
     const double dx = ((oct & 1) ? 0.25 : -0.25), dy = (((oct >> 1) & 1) ? 0.25 : -0.25),
@@ -1804,6 +1812,8 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
         W1[f] = static_cast<float>(a[e]);
         W2[f] = static_cast<float>(b[e]);
       }
+    split_bf16_parts(a, W1b.data() + static_cast<size_t>(oct) * 3 * KP * KP);
+    split_bf16_parts(b, W2b.data() + static_cast<size_t>(oct) * 3 * KP * KP);
   }
   const std::vector<int3> nf = near_offsets_field(), nh = near_offsets_hess();
   if (static_cast<int>(nf.size()) != kNearF || static_cast<int>(nh.size()) != kNearH)
@@ -1817,6 +1827,8 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
   MF_TRY(upload(t->Tb, Tb.data(), Tb.size() * sizeof(uint16_t)));
   MF_TRY(upload(t->W_m2m, W1.data(), W1.size() * sizeof(float)));
   MF_TRY(upload(t->W_l2l, W2.data(), W2.size() * sizeof(float)));
+  MF_TRY(upload(t->W_m2m_b, W1b.data(), W1b.size() * sizeof(uint16_t)));
+  MF_TRY(upload(t->W_l2l_b, W2b.data(), W2b.size() * sizeof(uint16_t)));
   MF_TRY(upload(t->slot_tab, slots.data(), slots.size() * sizeof(int32_t)));
   MF_TRY(upload(t->offsF, hof.data(), hof.size() * sizeof(int3)));
   MF_TRY(upload(t->offsH, hoh.data(), hoh.size() * sizeof(int3)));
@@ -2395,30 +2407,54 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
                        Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, F.Mt.as<float>());
     MF_LAUNCH_CHECK();
   }
+  // M2M and L2L run on the M2L kernel, in the M2L's precision: FP32 MFMA, or
+  // the BF16 parts of the octant matrices against BF16 parts of the source
+  // expansions, which are split level by level just before they are read.
+  const int prec = plan->m2l_precision();
+  const int npart = prec == MF_M2L_FP32 ? 0 : (prec == MF_M2L_BF16X6 ? 3 : 2);
+  auto split_level = [&](const float *src, int l) -> Status {
+    const int64_t ne = static_cast<int64_t>(F.count[l]) * KP;
+    if (ne == 0) return Status::Ok();
+    const int64_t off = static_cast<int64_t>(F.base[l]) * KP;
+    hipLaunchKernelGGL(split_multipoles_kernel, dim3(grid_for(ne, 256)), dim3(256), 0,
+                       ctx->stream, ne, KP, src + off, F.Mb.as<uint16_t>() + 3 * off);
+    MF_LAUNCH_CHECK();
+    return Status::Ok();
+  };
+  auto octant_pass = [&](const void *msrc, float *out, const void *tables, const M2LTile *tiles, int nt,
+                         const int32_t *boxes, const int32_t *src, int src_ld,
+                         const int32_t *tlist, const int32_t *tn, int accumulate) -> Status {
+    const dim3 grid(8 * ((nt + 7) / 8)), block(2 * KP);
+#define MF_OCTANT_ARGS                                                                          \
+  ctx->stream, msrc, out, tables, tiles, nt, boxes, src, src_ld,                               \
+      tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate
+    if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_OCTANT_ARGS);
+    else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_OCTANT_ARGS);
+    else hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_OCTANT_ARGS);
+#undef MF_OCTANT_ARGS
+    MF_LAUNCH_CHECK();
+    return Status::Ok();
+  };
   {
-    // M2M on the M2L kernel (FP32 MFMA): parents of level l gather their
-    // children's multipoles per octant and apply that octant's matrix.
+    // M2M: parents of level l gather their children's multipoles per octant
+    // and apply that octant's matrix.
     KernelTimer timer(ctx, "m2m");
     for (int l = D - 1; l >= 2; --l) {
+      if (npart) MF_TRY(split_level(F.Mt.as<float>(), l + 1));
       const int t0 = F.tileM_start[l], nt = F.tileM_start[l + 1] - t0;
       if (nt <= 0) continue;
-      hipLaunchKernelGGL((m2l_kernel<KP, 0>), dim3(8 * ((nt + 7) / 8)), dim3(2 * KP), 0,
-                         ctx->stream, F.Mt.get(), F.Mt.as<float>(), tab.W_m2m.get(),
+      MF_TRY(octant_pass(npart ? F.Mb.get() : F.Mt.get(), F.Mt.as<float>(),
+                         npart ? tab.W_m2m_b.get() : tab.W_m2m.get(),
                          F.tilesM.as<M2LTile>() + t0, nt, F.box_ids.as<int32_t>(),
-                         F.child_tab.as<int32_t>(), 8, tab.slot_tab.as<int32_t>(), tab.slot_ld,
+                         F.child_tab.as<int32_t>(), 8,
                          F.tlistM.as<int32_t>() + static_cast<int64_t>(t0) * tab.slot_ld,
-                         F.tnM.as<int32_t>() + t0, tab.slot_ld, 0);
-      MF_LAUNCH_CHECK();
+                         F.tnM.as<int32_t>() + t0, 0));
     }
   }
-  const int prec = plan->m2l_precision();
-  if (prec != MF_M2L_FP32) {
-    // split every multipole once; the M2L gathers each one many times
+  if (npart) {
+    // levels D .. 3 were split for the M2M; the M2L also reads level 2
     KernelTimer timer(ctx, "split");
-    const int64_t ne = static_cast<int64_t>(F.nbox) * KP;
-    hipLaunchKernelGGL(split_multipoles_kernel, dim3(grid_for(ne, 256)), dim3(256), 0,
-                       ctx->stream, ne, KP, F.Mt.as<float>(), F.Mb.as<uint16_t>());
-    MF_LAUNCH_CHECK();
+    MF_TRY(split_level(F.Mt.as<float>(), 2));
   }
   MF_TRY(before_m2l());
   {
@@ -2439,19 +2475,20 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     }
   }
   {
-    // L2L on the M2L kernel (FP32 MFMA, accumulating): the children of level
-    // l (tiles of one octant class) add their parent's local expansion.
+    // L2L (accumulating): the children of level l (tiles of one octant class)
+    // add their parent's local expansion.  With BF16 parts, the parents'
+    // locals are split into the multipole-part buffer, which the M2L no
+    // longer needs.
     KernelTimer timer(ctx, "l2l");
     for (int l = 3; l <= D; ++l) {
       const int t0 = F.tileL_start[l], nt = F.tileL_start[l + 1] - t0;
       if (nt <= 0) continue;
-      hipLaunchKernelGGL((m2l_kernel<KP, 0>), dim3(8 * ((nt + 7) / 8)), dim3(2 * KP), 0,
-                         ctx->stream, F.Lt.get(), F.Lt.as<float>(), tab.W_l2l.get(),
-                         F.tilesL.as<M2LTile>() + t0, nt, F.class_ids2.as<int32_t>(),
-                         F.box_parent.as<int32_t>(), 1, tab.slot_tab.as<int32_t>(), tab.slot_ld,
+      if (npart) MF_TRY(split_level(F.Lt.as<float>(), l - 1));
+      MF_TRY(octant_pass(npart ? F.Mb.get() : F.Lt.get(), F.Lt.as<float>(),
+                         npart ? tab.W_l2l_b.get() : tab.W_l2l.get(), F.tilesL.as<M2LTile>() + t0,
+                         nt, F.class_ids2.as<int32_t>(), F.box_parent.as<int32_t>(), 1,
                          F.tlistL.as<int32_t>() + static_cast<int64_t>(t0) * tab.slot_ld,
-                         F.tnL.as<int32_t>() + t0, tab.slot_ld, 1);
-      MF_LAUNCH_CHECK();
+                         F.tnL.as<int32_t>() + t0, 1));
     }
   }
   return Status::Ok();
