@@ -49,6 +49,7 @@ typedef enum mf_precision {
   MF_PREC_FP64 = 1       /* FP64 throughout (the GPU FP64 check path) */
 } mf_precision;
 
+/* Arithmetic of the FMM translations (M2L, and the M2M and L2L passes with it). */
 typedef enum mf_m2l_precision {
   MF_M2L_FP32 = 0,   /* FP32 MFMA (v_mfma_f32_16x16x4_f32) */
   MF_M2L_BF16X6 = 1, /* BF16 MFMA, operands split in 3 BF16 parts, 6 products,
