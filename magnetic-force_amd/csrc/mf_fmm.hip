@@ -491,7 +491,10 @@ __global__ void split_multipoles_kernel(int64_t nelem, int KP, const float *__re
   for (int k = 0; k < 3; ++k) Mb[(box * 3 + k) * KP + d] = static_cast<uint16_t>(p[k]);
 }
 
-template <int KP, int NPART>
+// OCTANT marks the instances that run the M2M and L2L passes (octant
+// matrices).  The code is the same; the separate instances let kernel traces
+// list those passes apart from the M2L.
+template <int KP, int NPART, bool OCTANT = false>
 __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Msrc,
                                                           float *__restrict__ Lt,
                                                           const void *__restrict__ Tv,
@@ -2428,9 +2431,9 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
 #define MF_OCTANT_ARGS                                                                          \
   ctx->stream, msrc, out, tables, tiles, nt, boxes, src, src_ld,                               \
       tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate
-    if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_OCTANT_ARGS);
-    else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_OCTANT_ARGS);
-    else hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_OCTANT_ARGS);
+    if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
+    else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2, true>), grid, block, 0, MF_OCTANT_ARGS);
+    else hipLaunchKernelGGL((m2l_kernel<KP, 3, true>), grid, block, 0, MF_OCTANT_ARGS);
 #undef MF_OCTANT_ARGS
     MF_LAUNCH_CHECK();
     return Status::Ok();
