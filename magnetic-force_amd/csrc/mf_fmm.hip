@@ -198,6 +198,22 @@ __global__ void run_flags_kernel(int64_t n, const uint64_t *__restrict__ keys, i
   flags[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1 : 0;
 }
 
+// Box counts of the levels above the leaves, from the sorted leaf keys: level
+// l < D holds the distinct prefixes key >> 3 (D - l), so counts[l] (zero on
+// entry) receives the number of prefix changes; the first box is the + 1.
+__global__ void level_counts_kernel(int nleaf, const uint64_t *__restrict__ keys, int D,
+                                    int32_t *__restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i > 0 && i < nleaf;
+  const uint64_t k = live ? keys[i] : 0ULL, kp = live ? keys[i - 1] : 0ULL;
+  const int lane = threadIdx.x & 63;
+  for (int l = 0; l < D; ++l) {
+    const int sh = 3 * (D - l);
+    const uint64_t m = __ballot(live && (k >> sh) != (kp >> sh));
+    if (m != 0 && lane == __ffsll(static_cast<long long>(m)) - 1) atomicAdd(&counts[l], __popcll(m));
+  }
+}
+
 // Writes the distinct keys and the index of each run's first element.
 __global__ void run_starts_kernel(int64_t n, const uint64_t *__restrict__ keys, int shift,
                                   const int32_t *__restrict__ flags, const int32_t *__restrict__ scan,
@@ -2021,22 +2037,31 @@ Status fmm_build(mf_plan *plan) {
   hipLaunchKernelGGL(run_counts_kernel, dim3(grid_for(cnt[D], 256)), dim3(256), 0, ctx->stream,
                      cnt[D], static_cast<int>(n), F.box_first.as<int32_t>(), F.box_count.as<int32_t>());
   MF_LAUNCH_CHECK();
+  if (D > 0) {
+    // the box counts of all levels above the leaves, in one readback
+    void *pc = nullptr;
+    MF_TRY(workspace(ctx, "fmm_level_counts", sizeof(int32_t) * D, &pc));
+    MF_HIP(hipMemsetAsync(pc, 0, sizeof(int32_t) * D, ctx->stream));
+    hipLaunchKernelGGL(level_counts_kernel, dim3(grid_for(cnt[D], 256)), dim3(256), 0, ctx->stream,
+                       cnt[D], F.box_keys.as<uint64_t>(), D, static_cast<int32_t *>(pc));
+    MF_LAUNCH_CHECK();
+    std::vector<int32_t> lc;
+    MF_TRY(read_ints(ctx, static_cast<const int32_t *>(pc), D, lc));
+    for (int l = 0; l < D; ++l) cnt[l] = lc[l] + 1;
+  }
+  // level-local child run of each child box (the largest level first, so the
+  // workspace does not grow inside the loop)
+  void *pr = nullptr;
+  MF_TRY(workspace(ctx, "fmm_child_run", sizeof(int32_t) * (cnt[D] + 1), &pr));
   for (int l = D - 1; l >= 0; --l) {
     const int nc = cnt[l + 1];
     const uint64_t *ck = F.box_keys.as<uint64_t>() + F.base[l + 1];
-    MF_TRY(need<int32_t>(F.flags, std::max<int64_t>(n, nc) + 1));
     hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream,
                        static_cast<int64_t>(nc), ck, 3, F.flags.as<int32_t>());
     MF_LAUNCH_CHECK();
     MF_TRY(exclusive_scan_i32(ctx, F.flags.as<int32_t>(), F.scan.as<int32_t>(), nc));
-    MF_HIP(hipMemcpyAsync(&last[0], F.scan.as<int32_t>() + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    MF_HIP(hipMemcpyAsync(&last[1], F.flags.as<int32_t>() + nc - 1, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    MF_HIP(hipStreamSynchronize(ctx->stream));
-    cnt[l] = last[0] + last[1];
     F.base[l] = cursor;
     // parent keys/first child (level-local child index), child -> parent
-    void *pr = nullptr;
-    MF_TRY(workspace(ctx, "fmm_child_run", sizeof(int32_t) * (nc + 1), &pr));
     hipLaunchKernelGGL(run_starts_kernel, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream,
                        static_cast<int64_t>(nc), ck, 3, F.flags.as<int32_t>(), F.scan.as<int32_t>(),
                        F.box_keys.as<uint64_t>() + cursor, F.box_first.as<int32_t>() + cursor,
