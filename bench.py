@@ -33,6 +33,7 @@ sys.path.insert(0, os.path.join(ROOT, "magnetic-force_amd"))
 
 FP32_PEAK_TFLOPS = 157.3        # MI355X published FP32 vector / FP32 MFMA peak
 BF16_PEAK_TFLOPS = 2500.0       # MI355X published dense BF16 MFMA peak
+HBM_PEAK_GBS = 8000.0           # MI355X published HBM3E bandwidth
 FIELD_FLOPS_PER_PAIR = 32       # far-branch field pair, FMA = 2, rsqrt = 1
 FORCE_FLOPS_PER_PAIR = 64       # far-branch 6-component force tensor pair
 
@@ -341,8 +342,9 @@ def run_droplets(ctx, args, m2l_code):
     from magforce import tune
     big = max(systems, key=lambda sy: sy.n)
     best, cands = tune.autotune_fmm(ctx, to_dev(big.pos), big.h, target_rel_err=1e-5,
-                                    precisions=(m2l_code,))
+                                    precisions=(m2l_code,), leaf_sides=(2.0, 2.5, 3.0))
     order = best.order if best else 10
+    leaf_side = best.leaf_side if best else 2.0
     tuned = {"on": big.name, "n": big.n, "chosen": best.as_dict() if best else None,
              "candidates": [c.as_dict() for c in cands]}
     log(f"droplets96 autotune: {tuned}")
@@ -350,7 +352,7 @@ def run_droplets(ctx, args, m2l_code):
     def step():
         if state["plan"] is None:
             state["plan"] = magforce.BatchPlan(ctx, pos, off, systems[0].h, order=order,
-                                               m2l_precision=m2l_code)
+                                               m2l_precision=m2l_code, leaf_side=leaf_side)
         else:
             state["plan"].update_positions(pos)
         res = state["plan"].solve(kappas, b, rtol=args.rtol, graph_iters=2)
@@ -370,7 +372,8 @@ def run_droplets(ctx, args, m2l_code):
     st = state["stats"]
     out = {"workload": "droplets96", "systems": len(systems), "n": off[-1],
            "n_min": min(sy.n for sy in systems), "n_max": max(sy.n for sy in systems),
-           "ms_per_step": ms, "m2l_precision": args.m2l, "order": order, "autotune": tuned,
+           "ms_per_step": ms, "m2l_precision": args.m2l, "order": order,
+           "leaf_side_h": leaf_side, "autotune": tuned,
            "graph_iters": 2,
            "solver": {"iterations": st["iterations"], "matvecs": st["matvecs"],
                       "converged": st["converged"],
@@ -574,17 +577,29 @@ def main():
 
     if mode == magforce.MODE_FMM:
         # P2P: direct pairs x 32 FLOPs against the FP32 vector peak.  M2L:
-        # 2 (p+1)^4 FLOPs per box pair (dense (p+1)^2 x (p+1)^2 real operator,
-        # padding excluded) against the MFMA peak of the instruction used; the
-        # BF16-split kernels issue 6 (or 3) BF16 products per FP32 product, so
-        # their FP32-equivalent peak is the BF16 peak / 6 (or / 3).
+        # algorithmic work 2 (p+1)^4 FLOPs per box pair (dense (p+1)^2 x (p+1)^2
+        # real operator, padding excluded) against the published peak of the
+        # MFMA instruction the kernel issues (FP32 MFMA 157.3, BF16 dense 2500
+        # TFLOP/s).  The MFMA work actually issued (64-target tiles, KP-padded
+        # operators, 3 or 6 BF16 products per FP32 product) and the algorithmic
+        # bytes per launch are reported beside it.
         order = info["order"]
+        kp = ((order + 1) ** 2 + 31) // 32 * 32
+        nprod = {"fp32": 1, "bf16x6": 6, "bf16x3": 3}[args.m2l]
         m2l_flops = float(info["m2l_field_pairs"]) * 2.0 * (order + 1) ** 4
-        m2l_peak = {"fp32": FP32_PEAK_TFLOPS, "bf16x6": BF16_PEAK_TFLOPS / 6,
-                    "bf16x3": BF16_PEAK_TFLOPS / 3}[args.m2l]
+        m2l_peak = FP32_PEAK_TFLOPS if args.m2l == "fp32" else BF16_PEAK_TFLOPS
+        m2l = kernel_roofline("m2l", m2l_flops, m2l_peak, "mfma")
+        issued = float(info["m2l_field_tile_slots"]) * 64 * kp * kp * 2 * nprod
+        m2l["issued_mfma_flops_per_launch"] = issued
+        m2l["issued_mfma_frac"] = (issued / (m2l["avg_launch_ms"] * 1e-3) / 1e12 / m2l_peak
+                                   if m2l["avg_launch_ms"] > 0 else 0.0)
+        # each source multipole read once, each local written once, each distinct
+        # offset operator read once; 4 bytes per coefficient (FP32, or two BF16 parts)
+        m2l["algorithmic_bytes_per_launch"] = (2.0 * info["boxes"] * (order + 1) ** 2 * 4
+                                               + float(info["m2l_field_operators"])
+                                               * (order + 1) ** 4 * 4)
         cands = [kernel_roofline("p2p_field", float(info["p2p_field_pairs"]) * FIELD_FLOPS_PER_PAIR,
-                                 FP32_PEAK_TFLOPS, "valu"),
-                 kernel_roofline("m2l", m2l_flops, m2l_peak, "mfma")]
+                                 FP32_PEAK_TFLOPS, "valu"), m2l]
         cands.sort(key=lambda r: -r["ms_per_step"])
         roofline = cands[0]
         roofline["others"] = cands[1:]
@@ -613,6 +628,25 @@ def main():
                 if r["kernel"] in ks:
                     r["traffic"] = ks[r["kernel"]]["hbm_bytes_per_launch"]
                     r["traffic_source"] = os.path.relpath(traffic_path, ROOT)
+    for r in [roofline] + roofline.get("others", []):
+        if r["kernel"] != "m2l":
+            continue
+        # The bound follows the measurements: the fraction of HBM bandwidth the
+        # counted traffic takes against the fraction of the MFMA peak the issued
+        # MFMA work takes.  Under an HBM bound, achieved and peak are restated as
+        # algorithmic bytes per launch / launch time against 8 TB/s.
+        t_s = r["avg_launch_ms"] * 1e-3
+        alg_b = r["algorithmic_bytes_per_launch"]
+        if r.get("traffic") and t_s > 0:
+            r["traffic_ratio"] = r["traffic"] / alg_b
+            r["hbm_frac_counted"] = r["traffic"] / t_s / (HBM_PEAK_GBS * 1e9)
+            if r["hbm_frac_counted"] > r["issued_mfma_frac"]:
+                r["bound"] = "hbm"
+                r["algorithmic_tflops"] = r["achieved"]
+                r["achieved"] = alg_b / t_s / 1e9
+                r["peak"] = HBM_PEAK_GBS
+                r["unit"] = "GB/s"
+                r["frac"] = r["achieved"] / HBM_PEAK_GBS
 
     compare = None
     if mode == magforce.MODE_FMM and args.m2l != "fp32" and not args.no_compare:

@@ -13,7 +13,10 @@
 //    FP32 partials on the host);
 //  * cudaStatus is a hipError_t; failures of the C ABI are thrown as
 //    std::runtime_error (the reference stored and ignored them);
-//  * only T = float is provided (the only instantiation the solver uses).
+//  * T is float, double or int.  float and double have every operation; int
+//    holds data (copies, import/export, reserve, add_to_element) and converts
+//    to the other types with the templated conversion operator
+//    (cuda_abstract_vector.cuh:202, 367-379), provided for all three.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -46,11 +49,46 @@ inline void check_hip(hipError_t e, const char *what) {
   if (e != hipSuccess)
     throw std::runtime_error(std::string("magforce: ") + what + ": " + hipGetErrorString(e));
 }
+
+// Element type code of mf_vec_convert.
+template <typename T> struct dtype_of;
+template <> struct dtype_of<float> { static constexpr int value = MF_DTYPE_F32; };
+template <> struct dtype_of<double> { static constexpr int value = MF_DTYPE_F64; };
+template <> struct dtype_of<int> { static constexpr int value = MF_DTYPE_I32; };
+
+// Arithmetic of the floating element types through the C ABI.
+template <typename T> struct vec_ops;
+template <> struct vec_ops<float> {
+  static int axpby(mf_ctx *c, int64_t n, float a, const float *ax, const float *ay,
+                   const float *az, float b, const float *bx, const float *by, const float *bz,
+                   float *ox, float *oy, float *oz) {
+    return mf_vec_axpby(c, n, a, ax, ay, az, b, bx, by, bz, ox, oy, oz);
+  }
+  static int dot(mf_ctx *c, int64_t n, const float *ax, const float *ay, const float *az,
+                 const float *bx, const float *by, const float *bz, double *r) {
+    return mf_vec_dot(c, n, ax, ay, az, bx, by, bz, r);
+  }
+};
+template <> struct vec_ops<double> {
+  static int axpby(mf_ctx *c, int64_t n, double a, const double *ax, const double *ay,
+                   const double *az, double b, const double *bx, const double *by,
+                   const double *bz, double *ox, double *oy, double *oz) {
+    return mf_vec_axpby_f64(c, n, a, ax, ay, az, b, bx, by, bz, ox, oy, oz);
+  }
+  static int dot(mf_ctx *c, int64_t n, const double *ax, const double *ay, const double *az,
+                 const double *bx, const double *by, const double *bz, double *r) {
+    return mf_vec_dot_f64(c, n, ax, ay, az, bx, by, bz, r);
+  }
+};
 }  // namespace detail
 
 template <typename T>
 class cuda_abstract_vector {
-  static_assert(std::is_same<T, float>::value, "cuda_abstract_vector is provided for float");
+  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value ||
+                    std::is_same<T, int>::value,
+                "cuda_abstract_vector is provided for float, double and int");
+  template <typename U>
+  friend class cuda_abstract_vector;
 
  public:
   typedef T value_type;
@@ -91,21 +129,37 @@ class cuda_abstract_vector {
     return *this;
   }
 
-  cuda_abstract_vector operator+(const cuda_abstract_vector &o) const { return combine(1.f, o, 1.f); }
-  cuda_abstract_vector operator-(const cuda_abstract_vector &o) const { return combine(1.f, o, -1.f); }
-  cuda_abstract_vector operator-() const { return scaled(-1.f); }
-  cuda_abstract_vector &operator+=(const cuda_abstract_vector &o) { return accumulate(o, 1.f); }
-  cuda_abstract_vector &operator-=(const cuda_abstract_vector &o) { return accumulate(o, -1.f); }
+  cuda_abstract_vector operator+(const cuda_abstract_vector &o) const { return combine(T(1), o, T(1)); }
+  cuda_abstract_vector operator-(const cuda_abstract_vector &o) const { return combine(T(1), o, T(-1)); }
+  cuda_abstract_vector operator-() const { return scaled(T(-1)); }
+  cuda_abstract_vector &operator+=(const cuda_abstract_vector &o) { return accumulate(o, T(1)); }
+  cuda_abstract_vector &operator-=(const cuda_abstract_vector &o) { return accumulate(o, T(-1)); }
   cuda_abstract_vector &operator*=(const T C) {
     mf_ctx *ctx = detail::default_ctx();
-    detail::check(mf_vec_axpby(ctx, static_cast<int64_t>(np), C, dev_x, dev_y, dev_z, 0.f, nullptr,
-                               nullptr, nullptr, dev_x, dev_y, dev_z),
+    detail::check(detail::vec_ops<T>::axpby(ctx, static_cast<int64_t>(np), C, dev_x, dev_y, dev_z,
+                                            T(0), nullptr, nullptr, nullptr, dev_x, dev_y, dev_z),
                   ctx);
     return *this;
   }
   const cuda_abstract_vector operator*(const T C) const { return scaled(C); }
+  // Element-wise conversion to another element type (cuda_abstract_vector.cuh:367-379):
+  // one mf_vec_convert launch per component; float or double to int truncates.
+  template <typename other_T>
+  operator cuda_abstract_vector<other_T>() const {
+    cuda_abstract_vector<other_T> r(np);
+    if (np == 0) return r;
+    mf_ctx *ctx = detail::default_ctx();
+    const T *src[3] = {dev_x, dev_y, dev_z};
+    other_T *dst[3] = {r.dev_x, r.dev_y, r.dev_z};
+    for (int c = 0; c < 3; ++c)
+      detail::check(mf_vec_convert(ctx, static_cast<int64_t>(np), detail::dtype_of<T>::value, src[c],
+                                   detail::dtype_of<other_T>::value, dst[c]),
+                    ctx);
+    return r;
+  }
 
   void import_data(const T *hx, const T *hy, const T *hz) {
+    if (np == 0) return;
     detail::check_hip(hipMemcpy(dev_x, hx, np * sizeof(T), hipMemcpyHostToDevice), "import_data");
     detail::check_hip(hipMemcpy(dev_y, hy, np * sizeof(T), hipMemcpyHostToDevice), "import_data");
     detail::check_hip(hipMemcpy(dev_z, hz, np * sizeof(T), hipMemcpyHostToDevice), "import_data");
@@ -127,6 +181,7 @@ class cuda_abstract_vector {
     import_data(hx.data(), hy.data(), hz.data());
   }
   void export_data(T *hx, T *hy, T *hz) const {
+    if (np == 0) return;
     detail::check_hip(hipMemcpy(hx, dev_x, np * sizeof(T), hipMemcpyDeviceToHost), "export_data");
     detail::check_hip(hipMemcpy(hy, dev_y, np * sizeof(T), hipMemcpyDeviceToHost), "export_data");
     detail::check_hip(hipMemcpy(hz, dev_z, np * sizeof(T), hipMemcpyDeviceToHost), "export_data");
@@ -168,8 +223,8 @@ class cuda_abstract_vector {
   double dot_fp64(const cuda_abstract_vector &o) const {
     double r = 0.0;
     mf_ctx *ctx = detail::default_ctx();
-    detail::check(mf_vec_dot(ctx, static_cast<int64_t>(np), dev_x, dev_y, dev_z, o.dev_x, o.dev_y,
-                             o.dev_z, &r),
+    detail::check(detail::vec_ops<T>::dot(ctx, static_cast<int64_t>(np), dev_x, dev_y, dev_z,
+                                          o.dev_x, o.dev_y, o.dev_z, &r),
                   ctx);
     return r;
   }
@@ -204,28 +259,31 @@ class cuda_abstract_vector {
     }
     detail::check_hip(cudaStatus, "hipMemcpy");
   }
-  cuda_abstract_vector combine(float a, const cuda_abstract_vector &o, float b) const {
+  cuda_abstract_vector combine(T a, const cuda_abstract_vector &o, T b) const {
     if (o.np != np) throw std::invalid_argument("cuda_abstract_vector: size mismatch");
     cuda_abstract_vector r(np);
     mf_ctx *ctx = detail::default_ctx();
-    detail::check(mf_vec_axpby(ctx, static_cast<int64_t>(np), a, dev_x, dev_y, dev_z, b, o.dev_x,
-                               o.dev_y, o.dev_z, r.dev_x, r.dev_y, r.dev_z),
+    detail::check(detail::vec_ops<T>::axpby(ctx, static_cast<int64_t>(np), a, dev_x, dev_y, dev_z,
+                                            b, o.dev_x, o.dev_y, o.dev_z, r.dev_x, r.dev_y,
+                                            r.dev_z),
                   ctx);
     return r;
   }
-  cuda_abstract_vector scaled(float a) const {
+  cuda_abstract_vector scaled(T a) const {
     cuda_abstract_vector r(np);
     mf_ctx *ctx = detail::default_ctx();
-    detail::check(mf_vec_axpby(ctx, static_cast<int64_t>(np), a, dev_x, dev_y, dev_z, 0.f, nullptr,
-                               nullptr, nullptr, r.dev_x, r.dev_y, r.dev_z),
+    detail::check(detail::vec_ops<T>::axpby(ctx, static_cast<int64_t>(np), a, dev_x, dev_y, dev_z,
+                                            T(0), nullptr, nullptr, nullptr, r.dev_x, r.dev_y,
+                                            r.dev_z),
                   ctx);
     return r;
   }
-  cuda_abstract_vector &accumulate(const cuda_abstract_vector &o, float b) {
+  cuda_abstract_vector &accumulate(const cuda_abstract_vector &o, T b) {
     if (o.np != np) throw std::invalid_argument("cuda_abstract_vector: size mismatch");
     mf_ctx *ctx = detail::default_ctx();
-    detail::check(mf_vec_axpby(ctx, static_cast<int64_t>(np), 1.f, dev_x, dev_y, dev_z, b, o.dev_x,
-                               o.dev_y, o.dev_z, dev_x, dev_y, dev_z),
+    detail::check(detail::vec_ops<T>::axpby(ctx, static_cast<int64_t>(np), T(1), dev_x, dev_y,
+                                            dev_z, b, o.dev_x, o.dev_y, o.dev_z, dev_x, dev_y,
+                                            dev_z),
                   ctx);
     return *this;
   }

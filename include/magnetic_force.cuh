@@ -37,18 +37,22 @@ namespace Libo {
 // Owns one mf_plan (a particle set with its all-pairs records or octree).
 class mf_evaluator {
  public:
-  mf_evaluator(int64_t n, const float *dx, const float *dy, const float *dz, float h, int mode);
+  // leaf_size: FMM particle budget per leaf (mf_plan_opts.leaf_size; 0 = leaf side 2h)
+  mf_evaluator(int64_t n, const float *dx, const float *dy, const float *dz, float h, int mode,
+               int leaf_size = 0);
   ~mf_evaluator();
   mf_evaluator(const mf_evaluator &) = delete;
   mf_evaluator &operator=(const mf_evaluator &) = delete;
   mf_plan *plan() const { return plan_; }
   int mode() const { return mode_; }
   float h() const { return h_; }
+  int leaf_size() const { return leaf_size_; }
 
  private:
   mf_plan *plan_ = nullptr;
   int mode_ = MF_MODE_ALLPAIRS;
   float h_ = 0.f;
+  int leaf_size_ = 0;
 };
 
 // cuda_magnet_cgOptimizer: magnetization solve and magnetic force for one

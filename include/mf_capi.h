@@ -54,8 +54,10 @@ typedef enum mf_m2l_precision {
   MF_M2L_FP32 = 0,   /* FP32 MFMA (v_mfma_f32_16x16x4_f32) */
   MF_M2L_BF16X6 = 1, /* BF16 MFMA, operands split in 3 BF16 parts, 6 products,
                         FP32 accumulation: FP32-level accuracy */
-  MF_M2L_BF16X3 = 2  /* BF16 MFMA, 2 parts, 3 products: ~1e-5 relative per
+  MF_M2L_BF16X3 = 2, /* BF16 MFMA, 2 parts, 3 products: ~1e-5 relative per
                         translation (inner products of the mixed solve) */
+  MF_M2L_BF16X1 = 3  /* BF16 MFMA, 1 part, 1 product: ~4e-3 relative per
+                        translation (inner corrections of the refined solve) */
 } mf_m2l_precision;
 
 typedef enum mf_mode {
@@ -156,13 +158,19 @@ int mf_force_subset_f64(mf_ctx *ctx, int64_t n, const double *x, const double *y
 /* ---- plans: one particle set, reused across products and solves ---------- */
 typedef struct mf_plan_opts {
   int mode;          /* mf_mode */
-  int leaf_size;     /* unused (the octree leaf side is fixed at 2h); must be 0 */
+  int leaf_size;     /* FMM: particle budget per leaf, the role of the reference's
+                        crit_size / ncrit (magnetic_force.cu:647-648, NDTree.hpp:413-496):
+                        with leaf_side 0, the leaf side is the largest 2h * 2^j whose
+                        non-empty leaves hold at most leaf_size particles on average.
+                        0 = leaf side 2h.  The octree has uniform depth. */
   int order;         /* FMM: expansion order p (degrees 0..p): 6, 8 or 10; 0 = 10 */
   int m2l_precision; /* FMM: mf_m2l_precision of mf_plan_field / mf_plan_force */
   int serial_near;   /* FMM: 0 = the near field runs on a second stream beside the M2L
                         (default); 1 = on the context stream, after the upward pass (for
                         timing each kernel alone) */
-  int reserved[3];
+  float leaf_side;   /* FMM: leaf side in units of h, >= 2 (0 = from leaf_size).  Larger
+                        leaves move work from the M2L to the near field. */
+  int reserved[2];
 } mf_plan_opts;
 
 /* Replaces the construction of fmm_field_evaluator / fmm_hessian_evaluator
@@ -190,6 +198,11 @@ typedef struct mf_plan_info {
   int64_t p2p_force_pairs;  /* ... per force product */
   int64_t m2l_field_pairs;  /* box pairs translated per field product */
   int64_t m2l_force_pairs;
+  int64_t m2l_field_tile_slots; /* (target tile, offset) steps the M2L kernel runs per
+                                   field product: each applies one offset's operator
+                                   to a tile of up to 64 target boxes */
+  int64_t m2l_force_tile_slots;
+  int64_t m2l_field_operators;  /* distinct offset operators of the field M2L */
 } mf_plan_info;
 int mf_plan_get_info(const mf_plan *plan, mf_plan_info *info);
 
@@ -212,6 +225,10 @@ typedef struct mf_solve_opts {
   int graph_iters;   /* mf_plan_solve_batch: > 0 captures one CG iteration (after one
                         eager iteration) in a hipGraph and replays it graph_iters
                         times between convergence checks; 0 = plain launches */
+  int inner_m2l_precision; /* mf_plan_solve_f64 on an FMM plan: 1 + the mf_m2l_precision
+                              of the inner correction solves; 0 = the plan's own */
+  double inner_rtol; /* mf_plan_solve_f64: relative tolerance of each inner solve;
+                        0 = 1e-5 (all-pairs plans), 1e-3 (FMM plans) */
 } mf_solve_opts;
 
 typedef struct mf_solve_stats {
@@ -230,12 +247,22 @@ int mf_plan_solve(mf_plan *plan, double kappa, const float *bx, const float *by,
                   const float *bz, float *xx, float *xy, float *xz, const mf_solve_opts *opts,
                   mf_solve_stats *stats);
 
-/* Mixed-precision iterative refinement: the residual r = b - A x is formed
- * with the FP64 all-pairs product, corrections come from the FP32 plan
- * solver, x is kept in FP64.  b and x are device double arrays laid out as
- * one contiguous [3][n] block each (x component first).  Used to obtain the
- * FP64 reference solution (rtol 1e-12) that FP32 and FMM results are scored
- * against. */
+/* Mixed-precision iterative refinement with x and the residual
+ * r = b - A x accumulated in FP64; the corrections d (A d = r) come from the
+ * plan's own solver (CG to opts->inner_rtol).  b and x are device double
+ * arrays laid out as one contiguous [3][n] block each (x component first);
+ * x holds the start.
+ *  - All-pairs plans: the residual product is the FP64 all-pairs product, so
+ *    x reaches the FP64 solution (rtol 1e-12): the reference solution that
+ *    FP32 and FMM results are scored against.
+ *  - FMM plans: the residual product is the plan's FMM product at its own M2L
+ *    precision, applied to the FP32 rounding of kappa x; the inner solves run
+ *    at opts->inner_m2l_precision (e.g. BF16x1 under a BF16x3 or FP32 plan).
+ *    x converges to the solution of the plan's FMM operator, whose distance
+ *    from the exact solution is set by the expansion order.
+ * stats->refinements counts the correction sweeps, stats->matvecs every
+ * product (residual and inner).  No counterpart in the reference, whose solve
+ * stops at an RMS residual of 0.1 (magnetic_force.cu:1043-1095). */
 int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3n,
                       const mf_solve_opts *opts, mf_solve_stats *stats);
 
@@ -277,6 +304,32 @@ int mf_vec_dot(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const f
                const float *bx, const float *by, const float *bz, double *result);
 /* x[i] = v for every i (one component array). */
 int mf_vec_fill(mf_ctx *ctx, int64_t n, float v, float *x);
+
+/* The same four operations on double arrays, for cuda_abstract_vector<double>
+ * (the reference's vector is a template over T, cuda_abstract_vector.cuh:166).
+ * mf_vec_dot_f64 multiplies and sums in FP64, in the same fixed order. */
+int mf_vec_axpby_f64(mf_ctx *ctx, int64_t n, double alpha, const double *ax, const double *ay,
+                     const double *az, double beta, const double *bx, const double *by,
+                     const double *bz, double *ox, double *oy, double *oz);
+int mf_vec_mul_f64(mf_ctx *ctx, int64_t n, const double *ax, const double *ay, const double *az,
+                   const double *bx, const double *by, const double *bz, double *ox, double *oy,
+                   double *oz);
+int mf_vec_dot_f64(mf_ctx *ctx, int64_t n, const double *ax, const double *ay, const double *az,
+                   const double *bx, const double *by, const double *bz, double *result);
+int mf_vec_fill_f64(mf_ctx *ctx, int64_t n, double v, double *x);
+
+/* Element types of mf_vec_convert. */
+typedef enum mf_dtype {
+  MF_DTYPE_F32 = 0,
+  MF_DTYPE_F64 = 1,
+  MF_DTYPE_I32 = 2
+} mf_dtype;
+/* dst[i] = (dst type) src[i] for one component array of n elements: the
+ * element conversion of cuda_abstract_vector's templated conversion operator
+ * (cuda_conversion_kernel, cuda_abstract_vector.cuh:367-379).  Floating
+ * values convert to int32 by truncation toward zero, as a C cast. */
+int mf_vec_convert(mf_ctx *ctx, int64_t n, int src_dtype, const void *src, int dst_dtype,
+                   void *dst);
 
 #ifdef __cplusplus
 }

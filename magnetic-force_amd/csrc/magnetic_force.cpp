@@ -29,11 +29,12 @@ void fill3(cuda_abstract_vector<float> &v, float a, float b, float c) {
 }  // namespace
 
 mf_evaluator::mf_evaluator(int64_t n, const float *dx, const float *dy, const float *dz, float h,
-                           int mode)
-    : mode_(mode), h_(h) {
+                           int mode, int leaf_size)
+    : mode_(mode), h_(h), leaf_size_(leaf_size) {
   mf_ctx *ctx = detail::default_ctx();
   mf_plan_opts o{};
   o.mode = mode;
+  o.leaf_size = mode == MF_MODE_FMM ? leaf_size : 0;
   detail::check(mf_plan_create(ctx, n, dx, dy, dz, h, &o, &plan_), ctx);
 }
 
@@ -74,7 +75,9 @@ cuda_magnet_cgOptimizer::cuda_magnet_cgOptimizer(
   Hext_source_M.import_data(a.ext_source_M);
   zero(total_B_over_mu0);  // the reference left x uninitialised
   zero(magnetic_bodyforce);
-  crit_size = 0;  // FMM leaf size: 0 = chosen by the library
+  // FMM particle budget per leaf (the reference's ncrit, 384 there): 0 = leaves
+  // of side 2h, the library's default; a caller may set it before solve()
+  crit_size = 0;
   fmm_object_pos = a.target_pos;
   fmm_hext_pos = a.ext_source_pos;
   fmm_hext_charge = a.ext_source_M;
@@ -87,9 +90,9 @@ cuda_magnet_cgOptimizer::cuda_magnet_cgOptimizer(
 mf_evaluator &cuda_magnet_cgOptimizer::self_evaluator() {
   const int mode = naive ? MF_MODE_ALLPAIRS : MF_MODE_FMM;
   if (!fmm_field_evaluator || fmm_field_evaluator->mode() != mode ||
-      fmm_field_evaluator->h() != h) {
+      fmm_field_evaluator->h() != h || fmm_field_evaluator->leaf_size() != crit_size) {
     fmm_field_evaluator = std::make_shared<mf_evaluator>(static_cast<int64_t>(np), xyz.dev_x,
-                                                         xyz.dev_y, xyz.dev_z, h, mode);
+                                                         xyz.dev_y, xyz.dev_z, h, mode, crit_size);
     fmm_hessian_evaluator = fmm_field_evaluator;  // one plan serves field and force
     fmm_hferro_evaluator = fmm_field_evaluator;
   }

@@ -462,7 +462,7 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown plan mode");
     }
-    if (plan->opts.m2l_precision < MF_M2L_FP32 || plan->opts.m2l_precision > MF_M2L_BF16X3) {
+    if (plan->opts.m2l_precision < MF_M2L_FP32 || plan->opts.m2l_precision > MF_M2L_BF16X1) {
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
     }
@@ -496,7 +496,7 @@ int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const fl
     MF_ARG(offsets[nsys] <= INT32_MAX, "batch too large");
     MF_ARG(opts && opts->mode == MF_MODE_FMM, "batch plans need mode MF_MODE_FMM");
     MF_ARG(h > 0.f, "h must be > 0");
-    MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X3,
+    MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X1,
            "unknown M2L precision");
     MF_ARG(opts->serial_near == 0 || opts->serial_near == 1, "serial_near must be 0 or 1");
     MF_HIP(hipSetDevice(ctx->device));
@@ -635,8 +635,10 @@ int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3
   auto run = [&]() -> Status {
     const int64_t n = plan->n;
     MF_ARG(n == 0 || (b3n && x3n), "pointers are null");
-    MF_ARG(plan->opts.mode == MF_MODE_ALLPAIRS,
-           "mf_plan_solve_f64 needs an all-pairs plan (its residual is the FP64 all-pairs product)");
+    MF_ARG(plan->nsys == 1, "mf_plan_solve_f64 takes single-system plans");
+    if (opts)
+      MF_ARG(opts->inner_m2l_precision >= 0 && opts->inner_m2l_precision <= MF_M2L_BF16X1 + 1,
+             "inner_m2l_precision out of range");
     MF_HIP(hipSetDevice(plan->ctx->device));
     mf_solve_opts o{};
     o.rtol = 1e-12;
@@ -650,33 +652,37 @@ int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3
   return mf::finish(plan->ctx, run());
 }
 
-int mf_vec_axpby(mf_ctx *ctx, int64_t n, float alpha, const float *ax, const float *ay,
-                 const float *az, float beta, const float *bx, const float *by,
-                 const float *bz, float *ox, float *oy, float *oz) {
+}  // extern "C"
+
+namespace {
+template <class T>
+int vec_axpby_t(mf_ctx *ctx, int64_t n, T alpha, const T *ax, const T *ay, const T *az, T beta,
+                const T *bx, const T *by, const T *bz, T *ox, T *oy, T *oz) {
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0, "n must be >= 0");
     MF_ARG(n == 0 || (ax && ay && az && ox && oy && oz), "pointers are null");
     MF_ARG(!bx == !by && !by == !bz, "b must be all null or all non-null");
-    return mf::vec_axpby(ctx, n, alpha, ax, ay, az, beta, bx, by, bz, ox, oy, oz);
+    return mf::vec_axpby<T>(ctx, n, alpha, ax, ay, az, beta, bx, by, bz, ox, oy, oz);
   };
   return mf::finish(ctx, run());
 }
 
-int mf_vec_mul(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
-               const float *bx, const float *by, const float *bz, float *ox, float *oy,
-               float *oz) {
+template <class T>
+int vec_mul_t(mf_ctx *ctx, int64_t n, const T *ax, const T *ay, const T *az, const T *bx,
+              const T *by, const T *bz, T *ox, T *oy, T *oz) {
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0, "n must be >= 0");
     MF_ARG(n == 0 || (ax && ay && az && bx && by && bz && ox && oy && oz), "pointers are null");
-    return mf::vec_mul(ctx, n, ax, ay, az, bx, by, bz, ox, oy, oz);
+    return mf::vec_mul<T>(ctx, n, ax, ay, az, bx, by, bz, ox, oy, oz);
   };
   return mf::finish(ctx, run());
 }
 
-int mf_vec_dot(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
-               const float *bx, const float *by, const float *bz, double *result) {
+template <class T>
+int vec_dot_t(mf_ctx *ctx, int64_t n, const T *ax, const T *ay, const T *az, const T *bx,
+              const T *by, const T *bz, double *result) {
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(result, "result is null");
@@ -686,17 +692,74 @@ int mf_vec_dot(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const f
       return Status::Ok();
     }
     MF_ARG(ax && ay && az && bx && by && bz, "pointers are null");
-    return mf::vec_dot(ctx, n, ax, ay, az, bx, by, bz, result);
+    return mf::vec_dot<T>(ctx, n, ax, ay, az, bx, by, bz, result);
   };
   return mf::finish(ctx, run());
 }
 
-int mf_vec_fill(mf_ctx *ctx, int64_t n, float v, float *x) {
+template <class T>
+int vec_fill_t(mf_ctx *ctx, int64_t n, T v, T *x) {
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0, "n must be >= 0");
     MF_ARG(n == 0 || x, "x is null");
-    return mf::vec_fill(ctx, n, v, x);
+    return mf::vec_fill<T>(ctx, n, v, x);
+  };
+  return mf::finish(ctx, run());
+}
+}  // namespace
+
+extern "C" {
+
+int mf_vec_axpby(mf_ctx *ctx, int64_t n, float alpha, const float *ax, const float *ay,
+                 const float *az, float beta, const float *bx, const float *by,
+                 const float *bz, float *ox, float *oy, float *oz) {
+  return vec_axpby_t<float>(ctx, n, alpha, ax, ay, az, beta, bx, by, bz, ox, oy, oz);
+}
+
+int mf_vec_mul(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
+               const float *bx, const float *by, const float *bz, float *ox, float *oy,
+               float *oz) {
+  return vec_mul_t<float>(ctx, n, ax, ay, az, bx, by, bz, ox, oy, oz);
+}
+
+int mf_vec_dot(mf_ctx *ctx, int64_t n, const float *ax, const float *ay, const float *az,
+               const float *bx, const float *by, const float *bz, double *result) {
+  return vec_dot_t<float>(ctx, n, ax, ay, az, bx, by, bz, result);
+}
+
+int mf_vec_fill(mf_ctx *ctx, int64_t n, float v, float *x) {
+  return vec_fill_t<float>(ctx, n, v, x);
+}
+
+int mf_vec_axpby_f64(mf_ctx *ctx, int64_t n, double alpha, const double *ax, const double *ay,
+                     const double *az, double beta, const double *bx, const double *by,
+                     const double *bz, double *ox, double *oy, double *oz) {
+  return vec_axpby_t<double>(ctx, n, alpha, ax, ay, az, beta, bx, by, bz, ox, oy, oz);
+}
+
+int mf_vec_mul_f64(mf_ctx *ctx, int64_t n, const double *ax, const double *ay, const double *az,
+                   const double *bx, const double *by, const double *bz, double *ox, double *oy,
+                   double *oz) {
+  return vec_mul_t<double>(ctx, n, ax, ay, az, bx, by, bz, ox, oy, oz);
+}
+
+int mf_vec_dot_f64(mf_ctx *ctx, int64_t n, const double *ax, const double *ay, const double *az,
+                   const double *bx, const double *by, const double *bz, double *result) {
+  return vec_dot_t<double>(ctx, n, ax, ay, az, bx, by, bz, result);
+}
+
+int mf_vec_fill_f64(mf_ctx *ctx, int64_t n, double v, double *x) {
+  return vec_fill_t<double>(ctx, n, v, x);
+}
+
+int mf_vec_convert(mf_ctx *ctx, int64_t n, int src_dtype, const void *src, int dst_dtype,
+                   void *dst) {
+  if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
+  auto run = [&]() -> Status {
+    MF_ARG(n >= 0, "n must be >= 0");
+    MF_ARG(n == 0 || (src && dst), "pointers are null");
+    return mf::vec_convert(ctx, n, src_dtype, src, dst_dtype, dst);
   };
   return mf::finish(ctx, run());
 }

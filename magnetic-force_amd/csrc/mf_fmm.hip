@@ -59,6 +59,11 @@ namespace {
 
 constexpr int kNearF = 81;
 constexpr int kNearFMixed = 27;  // slots with |o|_inf <= 1 may hold q < 2 pairs
+// Force near set: the boxes less than two leaf sides away (every pair within
+// 4h for leaf sides >= 2h).  Its slots are ordered by that separation; the
+// leading ones that may hold pairs within 4h (at leaf side 2h the 125 boxes
+// of {-2..2}^3) run the near test, the rest (at 2h the 54 boxes exactly two
+// leaf sides away) take the far form with no test.
 constexpr int kNearH = 179;
 constexpr int kMaxLevels = 21;
 
@@ -639,8 +644,10 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
                 acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[1], acc[rb][cb], 0, 0, 0);
                 acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][2], b[0], acc[rb][cb], 0, 0, 0);
               }
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[1], acc[rb][cb], 0, 0, 0);
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[0], acc[rb][cb], 0, 0, 0);
+              if constexpr (NPART >= 2) {
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[1], acc[rb][cb], 0, 0, 0);
+                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[0], acc[rb][cb], 0, 0, 0);
+              }
               acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[0], acc[rb][cb], 0, 0, 0);
             }
           }
@@ -1359,6 +1366,43 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
 
 // P2P, force: F[perm] (+)= (sum over the leaf's near ranges of B[m_j]) . m_i.
 // Leaf shapes (T) as in p2p_field_kernel.
+// Force near-field sums over one contiguous source range, every STRIDE-th
+// source from j0.  NEAR: the range may hold pairs within 4h (fitted curves
+// below q = 4); otherwise every pair takes the far form, with no test.
+template <int STRIDE, bool NEAR>
+__device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int j0, int j1,
+                                            float tx, float ty, float tz, float nx, float ny,
+                                            float nz, const HessConsts &hc, const float *tab,
+                                            float &dx0, float &dy0, float &dz0, float &mx0,
+                                            float &my0, float &mz0, float &n0) {
+#pragma unroll 2
+  for (int j = j0; j < j1; j += STRIDE) {
+    const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
+    const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
+    const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    const float ir2 = ir * ir;
+    float Pc = ir2 * ir2 * ir;
+    float Rc = -5.f * Pc * ir2;
+    if constexpr (NEAR) {
+      if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
+    }
+    const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+    const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+    const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
+    const float e = fmaf(Rc, dm * dn, Pc * mn);
+    const float f = Pc * dn;
+    dx0 = fmaf(dx, e, dx0);
+    dy0 = fmaf(dy, e, dy0);
+    dz0 = fmaf(dz, e, dz0);
+    mx0 = fmaf(mx, f, mx0);
+    my0 = fmaf(my, f, my0);
+    mz0 = fmaf(mz, f, mz0);
+    n0 = fmaf(Pc, dm, n0);
+  }
+}
+
 template <int T>
 __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_kernel(
     const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
@@ -1383,7 +1427,8 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_ke
   const int s0 = start[b], cnt = count[b];
   int tb, te;
   p2p_target_span<T>(cnt, tb, te);
-  const int nrange = nr[b].x;
+  const int2 nrb = nr[b];
+  const int nrange = nrb.x + nrb.y;
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   const double sfac = 3.0 * kMu0 / (4.0 * kPi);
   if constexpr (T < 64) {
@@ -1480,30 +1525,12 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_ke
     for (int r = 0; r < nrange; ++r) {
       const int2 q = rg[r];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
-#pragma unroll 2
-      for (int j = q.x + g; j < q.y; j += G) {
-        const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
-        const float mx = rec[j].mx, my = rec[j].my, mz = rec[j].mz;
-        const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float ir2 = ir * ir;
-        float Pc = ir2 * ir2 * ir;
-        float Rc = -5.f * Pc * ir2;
-        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
-        const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-        const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
-        const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
-        const float e = fmaf(Rc, dm * dn, Pc * mn);
-        const float f = Pc * dn;
-        dx0 = fmaf(dx, e, dx0);
-        dy0 = fmaf(dy, e, dy0);
-        dz0 = fmaf(dz, e, dz0);
-        mx0 = fmaf(mx, f, mx0);
-        my0 = fmaf(my, f, my0);
-        mz0 = fmaf(mz, f, mz0);
-        n0 = fmaf(Pc, dm, n0);
-      }
+      if (r < nrb.x)
+        force_range<G, true>(rec, q.x + g, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0,
+                             mx0, my0, mz0, n0);
+      else
+        force_range<G, false>(rec, q.x + g, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0,
+                              mx0, my0, mz0, n0);
       ad0 += dx0;
       ad1 += dy0;
       ad2 += dz0;
@@ -1590,13 +1617,17 @@ int excess2(int x, int y, int z) {
   return e(x) + e(y) + e(z);
 }
 
+// Boxes whose nearest points lie less than two leaf sides apart: every pair
+// within 4h (leaf side 2h).  Ordered by that separation (excess2 is its square
+// in leaf units), so the boxes that may hold pairs within 4h form a prefix.
 std::vector<int3> near_offsets_hess() {
   std::vector<int3> v;
-  for (int x = -3; x <= 3; ++x)
-    for (int y = -3; y <= 3; ++y)
-      for (int z = -3; z <= 3; ++z)
-        if (excess2(x, y, z) <= 4) v.push_back(make_int3(x, y, z));
-  return v;  // 179
+  for (int e = 0; e <= 4; ++e)
+    for (int x = -3; x <= 3; ++x)
+      for (int y = -3; y <= 3; ++y)
+        for (int z = -3; z <= 3; ++z)
+          if (excess2(x, y, z) == e) v.push_back(make_int3(x, y, z));
+  return v;  // 179: 125 with excess2 <= 3, then 54 with excess2 == 4
 }
 
 bool in_near_field(int x, int y, int z) { return x * x + y * y + z * z <= 6; }
@@ -1634,6 +1665,8 @@ int off_code(int x, int y, int z) { return ((x + 8) * 17 + (y + 8)) * 17 + (z + 
 struct FmmTables {
   int P = 0, KP = 0;
   int nslotF = 0, nslotH = 0;
+  int noffF = 0;          // distinct offsets of the field M2L (all parities)
+  std::vector<int> nearH_excess;  // excess2 of each force near slot (ascending)
   DeviceBuffer T;         // [noff][KP x KP in FP32 fragment order]
   DeviceBuffer Tb;        // BF16 bits [noff][3 parts][KP x KP in fragment order]
   DeviceBuffer W_m2m;     // [8][KP x KP in FP32 fragment order]
@@ -1661,6 +1694,7 @@ struct FmmState {
   DeviceBuffer sys_lo;  // double [nsys][3]: root corner of each system
   DeviceBuffer chunks;  // int64 [nchunk][2]: record ranges of the bounding-box pass
   double L = 0.0, s_leaf = 0.0;
+  int nmixH = 0;  // leading force near slots that may hold pairs within 4h
   int64_t n = 0;
   std::shared_ptr<FmmTables> tab;
   std::shared_ptr<FieldTable> ftab;
@@ -1698,6 +1732,8 @@ struct FmmState {
   int64_t p2p_force_pairs = 0;
   int64_t m2l_pairs_field = 0;
   int64_t m2l_pairs_force = 0;
+  int64_t m2l_slots_field = 0;  // sum over M2L tiles of their non-empty offsets
+  int64_t m2l_slots_force = 0;
   bool counts_valid = false;
 };
 
@@ -1771,6 +1807,12 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
           uniq.push_back(o);
         }
       }
+  {
+    std::map<int, int> field_codes;
+    for (int b = 0; b < 8; ++b)
+      for (const int3 &o : of[b]) field_codes[off_code(o.x, o.y, o.z)] = 1;
+    t->noffF = static_cast<int>(field_codes.size());
+  }
   t->slot_ld = std::max(t->nslotF, t->nslotH);
   // rows: 0..7 field M2L per parity, 8..15 force leaf M2L, 16 M2M (slot k ->
   // octant k), 17..24 L2L of a child of octant k (slot 0 -> octant k)
@@ -1837,6 +1879,7 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
   const std::vector<int3> nf = near_offsets_field(), nh = near_offsets_hess();
   if (static_cast<int>(nf.size()) != kNearF || static_cast<int>(nh.size()) != kNearH)
     return Status::Err(MF_ERR_UNSUPPORTED, "near-set size mismatch");
+  for (const int3 &o : nh) t->nearH_excess.push_back(excess2(o.x, o.y, o.z));
   auto upload = [&](DeviceBuffer &buf, const void *src, size_t bytes) -> Status {
     MF_TRY(buf.reserve(bytes));
     MF_HIP(hipMemcpy(buf.get(), src, bytes, hipMemcpyHostToDevice));
@@ -1893,7 +1936,10 @@ Status read_ints(mf_ctx *ctx, const int32_t *d, int count, std::vector<int32_t> 
 
 }  // namespace
 
-Status fmm_build(mf_plan *plan) {
+namespace {
+
+// Builds the tree and every list with leaves of side s_leaf (>= 2h).
+Status fmm_build_at(mf_plan *plan, double s_leaf) {
   mf_ctx *ctx = plan->ctx;
   if (!plan->fmm) plan->fmm.reset(new FmmState());
   FmmState &F = *plan->fmm;
@@ -1963,14 +2009,21 @@ Status fmm_build(mf_plan *plan) {
   }
   ext = ext * (1.0 + 1e-6) + 1e-6;
   int D = 0;
-  while (2.0 * h * std::ldexp(1.0, D) < ext) ++D;
+  while (s_leaf * std::ldexp(1.0, D) < ext) ++D;
   int sys_bits = 0;
   while ((1LL << sys_bits) < nsys) ++sys_bits;
   if (D > kMaxLevels - 1 || 3 * D + sys_bits > 63)
     return Status::Err(MF_ERR_UNSUPPORTED, "particle set too wide for the octree depth");
   F.D = D;
-  F.L = 2.0 * h * std::ldexp(1.0, D);
-  F.s_leaf = 2.0 * h;
+  F.L = s_leaf * std::ldexp(1.0, D);
+  F.s_leaf = s_leaf;
+  {
+    // force near slots with separation (sqrt(excess2) leaf sides) below 4h
+    const double t = 4.0 * h / s_leaf;
+    F.nmixH = 0;
+    for (int e : tab.nearH_excess)
+      if (static_cast<double>(e) < t * t) ++F.nmixH;
+  }
   std::vector<double> root(3 * nsys);
   for (int sy = 0; sy < nsys; ++sy)
     for (int a = 0; a < 3; ++a)
@@ -2116,7 +2169,7 @@ Status fmm_build(mf_plan *plan) {
                      F.box_count.as<int32_t>(), F.rngF.as<int2>(), F.nrF.as<int2>());
   MF_LAUNCH_CHECK();
   hipLaunchKernelGGL(near_ranges_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                     F.nearH.as<int32_t>(), kNearH, kNearH, F.box_first.as<int32_t>(),
+                     F.nearH.as<int32_t>(), kNearH, F.nmixH, F.box_first.as<int32_t>(),
                      F.box_count.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>());
   MF_LAUNCH_CHECK();
   {
@@ -2314,6 +2367,31 @@ Status fmm_build(mf_plan *plan) {
   return Status::Ok();
 }
 
+}  // namespace
+
+// The leaf side: leaf_side * h when set; otherwise 2h, or with a particle
+// budget leaf_size (the role of the reference's ncrit, NDTree.hpp:413-496,
+// magnetic_force.cu:647-648) the largest 2h * 2^j whose non-empty leaves hold
+// at most leaf_size particles on average.  Any side >= 2h keeps every pair
+// with q < 2 (field) and q < 4 (force) inside the near sets.
+Status fmm_build(mf_plan *plan) {
+  const double h = plan->h;
+  const float side = plan->opts.leaf_side;
+  MF_ARG(side == 0.f || (side >= 2.f && side <= 1024.f), "leaf_side must be 0 or in [2, 1024] (units of h)");
+  MF_ARG(plan->opts.leaf_size >= 0, "leaf_size must be >= 0");
+  if (side > 0.f) return fmm_build_at(plan, static_cast<double>(side) * h);
+  MF_TRY(fmm_build_at(plan, 2.0 * h));
+  const FmmState &F = *plan->fmm;
+  if (plan->opts.leaf_size <= 0 || plan->n == 0) return Status::Ok();
+  int j = 0;
+  for (int jj = 1; jj <= F.D; ++jj) {
+    if (static_cast<double>(plan->n) / F.count[F.D - jj] > plan->opts.leaf_size) break;
+    j = jj;
+  }
+  if (j == 0) return Status::Ok();
+  return fmm_build_at(plan, 2.0 * h * std::ldexp(1.0, j));
+}
+
 Status fmm_counts(mf_plan *plan) {
   mf_ctx *ctx = plan->ctx;
   FmmState &F = *plan->fmm;
@@ -2358,6 +2436,14 @@ Status fmm_counts(mf_plan *plan) {
     F.m2l_pairs_field = static_cast<int64_t>(hc[2]);
     F.m2l_pairs_force = static_cast<int64_t>(hc[3]);
   }
+  {
+    std::vector<int32_t> nf, nh;
+    MF_TRY(read_ints(ctx, F.tnF.as<int32_t>(), F.ntilesF, nf));
+    MF_TRY(read_ints(ctx, F.tnH.as<int32_t>(), F.ntilesH, nh));
+    F.m2l_slots_field = F.m2l_slots_force = 0;
+    for (int32_t v : nf) F.m2l_slots_field += v;
+    for (int32_t v : nh) F.m2l_slots_force += v;
+  }
   F.counts_valid = true;
   return Status::Ok();
 }
@@ -2373,6 +2459,9 @@ Status fmm_info(mf_plan *plan, mf_plan_info *info) {
   info->p2p_force_pairs = F.p2p_force_pairs;
   info->m2l_field_pairs = F.m2l_pairs_field;
   info->m2l_force_pairs = F.m2l_pairs_force;
+  info->m2l_field_tile_slots = F.m2l_slots_field;
+  info->m2l_force_tile_slots = F.m2l_slots_force;
+  info->m2l_field_operators = F.tab ? F.tab->noffF : 0;
   return Status::Ok();
 }
 
@@ -2407,6 +2496,12 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
       hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
+    case MF_M2L_BF16X1: {
+      const void *tables = tab.Tb.get();
+      const void *msrc = F.Mb.get();
+      hipLaunchKernelGGL((m2l_kernel<KP, 1>), grid, block, 0, MF_M2L_ARGS);
+      break;
+    }
     default:
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
   }
@@ -2439,7 +2534,10 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
   // the BF16 parts of the octant matrices against BF16 parts of the source
   // expansions, which are split level by level just before they are read.
   const int prec = plan->m2l_precision();
-  const int npart = prec == MF_M2L_FP32 ? 0 : (prec == MF_M2L_BF16X6 ? 3 : 2);
+  const int npart = prec == MF_M2L_FP32 ? 0
+                    : prec == MF_M2L_BF16X6 ? 3
+                    : prec == MF_M2L_BF16X3 ? 2
+                                            : 1;
   auto split_level = [&](const float *src, int l) -> Status {
     const int64_t ne = static_cast<int64_t>(F.count[l]) * KP;
     if (ne == 0) return Status::Ok();
@@ -2457,6 +2555,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
   ctx->stream, msrc, out, tables, tiles, nt, boxes, src, src_ld,                               \
       tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
+    else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2, true>), grid, block, 0, MF_OCTANT_ARGS);
     else hipLaunchKernelGGL((m2l_kernel<KP, 3, true>), grid, block, 0, MF_OCTANT_ARGS);
 #undef MF_OCTANT_ARGS

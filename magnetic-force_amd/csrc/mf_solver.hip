@@ -248,6 +248,11 @@ __global__ void f64_axpy_f32_kernel(int64_t n3, double s, const float *__restric
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n3) x[i] += s * static_cast<double>(d[i]);
 }
+// g64 = (double) g32 (the FMM product of the refinement residual)
+__global__ void f32_to_f64_kernel(int64_t n3, const float *__restrict__ a, double *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n3) out[i] = static_cast<double>(a[i]);
+}
 
 }  // namespace
 
@@ -418,7 +423,7 @@ Status solve_gmres(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_opts
       MF_TRY(plan_apply_G(plan, static_cast<float>(kappa), vj, vj + n, vj + 2 * n, nullptr, w.g,
                           w.g + n, w.g + 2 * n));
       st->matvecs += 1;
-      MF_TRY(vec_axpby(ctx, n, 1.f, vj, vj + n, vj + 2 * n, -1.f, w.g, w.g + n, w.g + 2 * n, wv,
+      MF_TRY(vec_axpby<float>(ctx, n, 1.f, vj, vj + n, vj + 2 * n, -1.f, w.g, w.g + n, w.g + 2 * n, wv,
                        wv + n, wv + 2 * n));
       std::fill(hcol.begin(), hcol.end(), 0.0);
       for (int pass = 0; pass < 2; ++pass) {
@@ -495,23 +500,23 @@ Status solve_ref_nlcg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_o
   auto residual_fn = [&]() -> Status {
     MF_TRY(plan_apply_G(plan, k, x.x, x.y, x.z, nullptr, tmp, tmp + n, tmp + 2 * n));
     st->matvecs += 1;
-    MF_TRY(vec_axpby(ctx, n, 1.f, tmp, tmp + n, tmp + 2 * n, 1.f, b.x, b.y, b.z, res, res + n,
+    MF_TRY(vec_axpby<float>(ctx, n, 1.f, tmp, tmp + n, tmp + 2 * n, 1.f, b.x, b.y, b.z, res, res + n,
                      res + 2 * n));
-    return vec_axpby(ctx, n, 1.f, res, res + n, res + 2 * n, -1.f, x.x, x.y, x.z, res, res + n,
+    return vec_axpby<float>(ctx, n, 1.f, res, res + n, res + 2 * n, -1.f, x.x, x.y, x.z, res, res + n,
                      res + 2 * n);
   };
   // gradient = kappa G(res) - res
   auto gradient_fn = [&]() -> Status {
     MF_TRY(plan_apply_G(plan, 1.f, res, res + n, res + 2 * n, nullptr, tmp, tmp + n, tmp + 2 * n));
     st->matvecs += 1;
-    return vec_axpby(ctx, n, k, tmp, tmp + n, tmp + 2 * n, -1.f, res, res + n, res + 2 * n, grad,
+    return vec_axpby<float>(ctx, n, k, tmp, tmp + n, tmp + 2 * n, -1.f, res, res + n, res + 2 * n, grad,
                      grad + n, grad + 2 * n);
   };
   // alpha = -(D(sd).res)/(|D(sd)|^2 + 1e-10), D(v) = G(kappa v) - v
   auto alpha_fn = [&](double *alpha) -> Status {
     MF_TRY(plan_apply_G(plan, k, sd, sd + n, sd + 2 * n, nullptr, tmp, tmp + n, tmp + 2 * n));
     st->matvecs += 1;
-    MF_TRY(vec_axpby(ctx, n, 1.f, tmp, tmp + n, tmp + 2 * n, -1.f, sd, sd + n, sd + 2 * n, tmp,
+    MF_TRY(vec_axpby<float>(ctx, n, 1.f, tmp, tmp + n, tmp + 2 * n, -1.f, sd, sd + n, sd + 2 * n, tmp,
                      tmp + n, tmp + 2 * n));
     double nt = 0.0, tr = 0.0;
     MF_TRY(vec_dot(ctx, n, tmp, tmp + n, tmp + 2 * n, tmp, tmp + n, tmp + 2 * n, &nt));
@@ -538,10 +543,10 @@ Status solve_ref_nlcg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_o
   int i_step = 0, it = 0;
   if (std::sqrt(err / static_cast<double>(n)) >= tol) {
     MF_TRY(gradient_fn());
-    MF_TRY(vec_axpby(ctx, n, -1.f, grad, grad + n, grad + 2 * n, 0.f, nullptr, nullptr, nullptr,
+    MF_TRY(vec_axpby<float>(ctx, n, -1.f, grad, grad + n, grad + 2 * n, 0.f, nullptr, nullptr, nullptr,
                      sd, sd + n, sd + 2 * n));
     MF_TRY(alpha_fn(&alpha));
-    MF_TRY(vec_axpby(ctx, n, 1.f, x.x, x.y, x.z, static_cast<float>(alpha), sd, sd + n,
+    MF_TRY(vec_axpby<float>(ctx, n, 1.f, x.x, x.y, x.z, static_cast<float>(alpha), sd, sd + n,
                      sd + 2 * n, x.x, x.y, x.z));
     for (; it < max_iteration; ++it) {
       MF_HIP(hipMemcpyAsync(ograd, grad, sizeof(float) * 3 * n, hipMemcpyDeviceToDevice, ctx->stream));
@@ -557,16 +562,16 @@ Status solve_ref_nlcg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_o
       if (i_step % 10 != 0) {
         double gg = 0.0, den = 0.0;
         MF_TRY(vec_dot(ctx, n, grad, grad + n, grad + 2 * n, grad, grad + n, grad + 2 * n, &gg));
-        MF_TRY(vec_axpby(ctx, n, 1.f, ograd, ograd + n, ograd + 2 * n, -1.f, grad, grad + n,
+        MF_TRY(vec_axpby<float>(ctx, n, 1.f, ograd, ograd + n, ograd + 2 * n, -1.f, grad, grad + n,
                          grad + 2 * n, tmp, tmp + n, tmp + 2 * n));
         MF_TRY(vec_dot(ctx, n, osd, osd + n, osd + 2 * n, tmp, tmp + n, tmp + 2 * n, &den));
         beta = -gg / (den + 1e-10);
         if (beta < 0.0) beta = 0.0;
       }
-      MF_TRY(vec_axpby(ctx, n, -1.f, grad, grad + n, grad + 2 * n, static_cast<float>(beta), osd,
+      MF_TRY(vec_axpby<float>(ctx, n, -1.f, grad, grad + n, grad + 2 * n, static_cast<float>(beta), osd,
                        osd + n, osd + 2 * n, sd, sd + n, sd + 2 * n));
       MF_TRY(alpha_fn(&alpha));
-      MF_TRY(vec_axpby(ctx, n, 1.f, x.x, x.y, x.z, static_cast<float>(alpha), sd, sd + n,
+      MF_TRY(vec_axpby<float>(ctx, n, 1.f, x.x, x.y, x.z, static_cast<float>(alpha), sd, sd + n,
                        sd + 2 * n, x.x, x.y, x.z));
     }
   }
@@ -625,8 +630,10 @@ Status solve(mf_plan *plan, double kappa, const float *bx, const float *by, cons
   return Status::Ok();
 }
 
-// Iterative refinement with an FP64 residual: r = b - A x in FP64 (all-pairs
-// FP64 product), correction from the FP32 plan solver, x kept in FP64.
+// Iterative refinement with x and r = b - A x in FP64; corrections from the
+// plan's own CG.  All-pairs plans form the residual with the FP64 all-pairs
+// product; FMM plans with the plan's FMM product (its own M2L precision), the
+// inner solves running at opts.inner_m2l_precision.
 Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by,
                  const double *bz, double *xx, double *xy, double *xz, const mf_solve_opts &opts,
                  mf_solve_stats *st) {
@@ -636,8 +643,8 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
   if (n == 0) return Status::Ok();
   MF_ARG(by == bx + n && bz == bx + 2 * n && xy == xx + n && xz == xx + 2 * n,
          "solve_f64 expects contiguous [3][n] arrays");
-  const double *px, *py, *pz;
-  MF_TRY(plan_pos64(plan, &px, &py, &pz));
+  const double *px = nullptr, *py = nullptr, *pz = nullptr;
+  if (plan->opts.mode != MF_MODE_FMM) MF_TRY(plan_pos64(plan, &px, &py, &pz));
   void *p = nullptr;
   MF_TRY(workspace(ctx, "f64_vecs", sizeof(double) * 3 * static_cast<size_t>(n3), &p));
   double *m64 = static_cast<double *>(p), *g64 = m64 + n3, *r64 = g64 + n3;
@@ -646,15 +653,38 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
   MF_TRY(workspace(ctx, "f64_partials", sizeof(double) * (kRedMaxBlocks + 8), &p));
   double *partials = static_cast<double *>(p), *dsum = partials + kRedMaxBlocks;
   const int nb = red_blocks(n3);
+  const bool fmm = plan->opts.mode == MF_MODE_FMM;
   const double rtol = opts.rtol > 0.0 ? opts.rtol : 1e-12;
   const int max_ref = opts.refine_fp64 > 0 ? opts.refine_fp64 : 10;
+  const double inner_rtol = opts.inner_rtol > 0.0 ? opts.inner_rtol : (fmm ? 1e-3 : 1e-5);
+  // FMM plans: the residual product runs at the plan's own M2L precision, the
+  // inner solves at inner_m2l_precision (the override is restored on every exit)
+  struct OverrideGuard {
+    mf_plan *p;
+    int saved;
+    ~OverrideGuard() { p->m2l_override = saved; }
+  } guard{plan, plan->m2l_override};
+  const int outer_prec = plan->m2l_override;
+  const int inner_prec = opts.inner_m2l_precision > 0 ? opts.inner_m2l_precision - 1 : outer_prec;
   double bn = 0.0;
   for (int ref = 0; ref <= max_ref; ++ref) {
-    hipLaunchKernelGGL(f64_moments_kernel, dim3(grid_for(n3, 256)), dim3(256), 0, ctx->stream, n,
-                       kappa, xx, m64);
-    MF_LAUNCH_CHECK();
-    MF_TRY(field_allpairs_f64(ctx, n, px, py, pz, n, px, py, pz, m64, m64 + n, m64 + 2 * n,
-                              plan->h, nullptr, g64, g64 + n, g64 + 2 * n));
+    if (fmm) {
+      // g = G (kappa x) from the FP32 rounding of kappa x, promoted to FP64
+      hipLaunchKernelGGL(f64_to_f32_scaled_kernel, dim3(grid_for(n3, 256)), dim3(256), 0,
+                         ctx->stream, n3, kappa, xx, rhs);
+      MF_LAUNCH_CHECK();
+      plan->m2l_override = outer_prec;
+      MF_TRY(plan_apply_G(plan, 1.f, rhs, rhs + n, rhs + 2 * n, nullptr, d, d + n, d + 2 * n));
+      hipLaunchKernelGGL(f32_to_f64_kernel, dim3(grid_for(n3, 256)), dim3(256), 0, ctx->stream,
+                         n3, d, g64);
+      MF_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(f64_moments_kernel, dim3(grid_for(n3, 256)), dim3(256), 0, ctx->stream, n,
+                         kappa, xx, m64);
+      MF_LAUNCH_CHECK();
+      MF_TRY(field_allpairs_f64(ctx, n, px, py, pz, n, px, py, pz, m64, m64 + n, m64 + 2 * n,
+                                plan->h, nullptr, g64, g64 + n, g64 + 2 * n));
+    }
     st->matvecs += 1;
     if (ref == 0) {
       hipLaunchKernelGGL(f64_norm2_kernel, dim3(nb), dim3(kRedBlock), 0, ctx->stream, n3, bx,
@@ -687,8 +717,9 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
     inner.method = MF_SOLVE_CG;
     inner.zero_start = 1;
     inner.maxit = opts.maxit > 0 ? opts.maxit : 500;
-    inner.rtol = 1e-5;
+    inner.rtol = inner_rtol;
     mf_solve_stats ist{};
+    if (fmm) plan->m2l_override = inner_prec;
     MF_TRY(solve(plan, kappa, rhs, rhs + n, rhs + 2 * n, d, d + n, d + 2 * n, inner, &ist));
     st->matvecs += ist.matvecs;
     st->iterations += ist.iterations;

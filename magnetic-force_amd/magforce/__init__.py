@@ -41,6 +41,7 @@ MODE_FMM = 1
 M2L_FP32 = 0
 M2L_BF16X6 = 1
 M2L_BF16X3 = 2
+M2L_BF16X1 = 3
 SOLVE_CG = 0
 SOLVE_GMRES = 1
 SOLVE_REF_NLCG = 2
@@ -55,7 +56,7 @@ class MagforceError(RuntimeError):
 class PlanOpts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int), ("leaf_size", ctypes.c_int), ("order", ctypes.c_int),
                 ("m2l_precision", ctypes.c_int), ("serial_near", ctypes.c_int),
-                ("reserved", ctypes.c_int * 3)]
+                ("leaf_side", ctypes.c_float), ("reserved", ctypes.c_int * 2)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -63,7 +64,9 @@ class PlanInfo(ctypes.Structure):
                 ("depth", ctypes.c_int), ("reserved0", ctypes.c_int), ("leaves", ctypes.c_int64),
                 ("boxes", ctypes.c_int64), ("p2p_field_pairs", ctypes.c_int64),
                 ("p2p_force_pairs", ctypes.c_int64), ("m2l_field_pairs", ctypes.c_int64),
-                ("m2l_force_pairs", ctypes.c_int64)]
+                ("m2l_force_pairs", ctypes.c_int64), ("m2l_field_tile_slots", ctypes.c_int64),
+                ("m2l_force_tile_slots", ctypes.c_int64),
+                ("m2l_field_operators", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved0"}
@@ -73,7 +76,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("method", ctypes.c_int), ("restart", ctypes.c_int), ("maxit", ctypes.c_int),
                 ("rtol", ctypes.c_double), ("refine_fp64", ctypes.c_int),
                 ("true_residual", ctypes.c_int), ("zero_start", ctypes.c_int),
-                ("graph_iters", ctypes.c_int)]
+                ("graph_iters", ctypes.c_int), ("inner_m2l_precision", ctypes.c_int),
+                ("inner_rtol", ctypes.c_double)]
 
 
 class SolveStats(ctypes.Structure):
@@ -353,13 +357,14 @@ class Plan:
 
     def __init__(self, ctx: Context, pos, h: float, mode: int = MODE_ALLPAIRS,
                  leaf_size: int = 0, order: int = 0, m2l_precision: int = 0,
-                 serial_near: bool = False):
+                 serial_near: bool = False, leaf_side: float = 0.0):
         self.ctx = ctx
         self.h = float(h)
         p = _soa(pos)
         self.n = p.shape[1]
         self.opts = PlanOpts(mode=mode, leaf_size=leaf_size, order=order,
-                             m2l_precision=m2l_precision, serial_near=int(bool(serial_near)))
+                             m2l_precision=m2l_precision, serial_near=int(bool(serial_near)),
+                             leaf_side=float(leaf_side))
         hp = ctypes.c_void_p()
         _check(lib().mf_plan_create(ctx.h, self.n, _ptr(p, 0), _ptr(p, 1), _ptr(p, 2), self.h,
                                     ctypes.byref(self.opts), ctypes.byref(hp)), ctx.h)
@@ -428,14 +433,20 @@ class Plan:
         return SolveResult(x, st.as_dict())
 
     def solve_f64(self, kappa: float, b, x0=None, rtol: float = 1e-12, max_refine: int = 10,
-                  maxit: int = 500):
-        """FP64 iterative refinement (all-pairs plans)."""
+                  maxit: int = 500, inner_m2l_precision=None, inner_rtol: float = 0.0):
+        """Iterative refinement with x and the residual in FP64.  All-pairs
+        plans: FP64 all-pairs residual (the FP64 solution).  FMM plans: the
+        residual from the plan's FMM product, inner corrections at
+        inner_m2l_precision (None = the plan's own)."""
         import torch
         if b.shape[0] != 3 and b.shape[1] == 3:
             b = b.t()
         bb = b.to(dtype=torch.float64).contiguous()
         x = torch.zeros_like(bb) if x0 is None else x0.to(torch.float64).contiguous().clone()
-        opts = SolveOpts(method=SOLVE_CG, maxit=maxit, rtol=rtol, refine_fp64=max_refine)
+        opts = SolveOpts(method=SOLVE_CG, maxit=maxit, rtol=rtol, refine_fp64=max_refine,
+                         inner_m2l_precision=0 if inner_m2l_precision is None
+                         else int(inner_m2l_precision) + 1,
+                         inner_rtol=float(inner_rtol))
         st = SolveStats()
         _check(lib().mf_plan_solve_f64(self.h_plan, float(kappa), _ptr(bb), _ptr(x),
                                        ctypes.byref(opts), ctypes.byref(st)), self.ctx.h)
@@ -448,7 +459,7 @@ class BatchPlan(Plan):
     block-diagonal; solve() runs the batched CG with one kappa per system."""
 
     def __init__(self, ctx: Context, pos, offsets, h: float, order: int = 0,
-                 m2l_precision: int = 0, serial_near: bool = False):
+                 m2l_precision: int = 0, serial_near: bool = False, leaf_side: float = 0.0):
         self.ctx = ctx
         self.h = float(h)
         p = _soa(pos)
@@ -459,7 +470,7 @@ class BatchPlan(Plan):
         self.offsets = off
         self.nsys = len(off) - 1
         self.opts = PlanOpts(mode=MODE_FMM, order=order, m2l_precision=m2l_precision,
-                             serial_near=int(bool(serial_near)))
+                             serial_near=int(bool(serial_near)), leaf_side=float(leaf_side))
         arr = (ctypes.c_int64 * len(off))(*off)
         hp = ctypes.c_void_p()
         _check(lib().mf_plan_create_batch(ctx.h, self.nsys, arr, _ptr(p, 0), _ptr(p, 1),

@@ -37,12 +37,31 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.mf_plan_size(None) == -1
 
 
-def test_struct_layouts_match_header():
-    # mf_plan_opts: 5 ints + 3 reserved; mf_solve_opts: 3 ints, double, 4 ints
-    assert ctypes.sizeof(magforce.PlanOpts) == 32
-    assert magforce.PlanOpts.serial_near.offset == 16
-    assert ctypes.sizeof(magforce.SolveOpts) == 40
-    assert ctypes.sizeof(magforce.SolveStats) == 32
+def test_struct_layouts_match_header(tmp_path):
+    """sizeof and every member offset of the header's structs, as the C
+    compiler lays them out, equal the ctypes mirror's."""
+    mirror = {"mf_plan_opts": magforce.PlanOpts, "mf_solve_opts": magforce.SolveOpts,
+              "mf_solve_stats": magforce.SolveStats, "mf_plan_info": magforce.PlanInfo}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "mf_capi.h"', 'int main(void) {']
+    for cname, py in mirror.items():
+        lines.append(f'  printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.dirname(magforce.HEADER_PATH)
+    subprocess.run(["gcc", "-std=c11", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    seen = 0
+    for line in out.splitlines():
+        cname, field, value = line.split()
+        py = mirror[cname]
+        want = ctypes.sizeof(py) if field == "sizeof" else getattr(py, field).offset
+        assert int(value) == want, (cname, field)
+        seen += 1
+    assert seen == sum(len(p._fields_) + 1 for p in mirror.values())
 
 
 def test_enum_constants_match_header():

@@ -263,6 +263,70 @@ def test_fmm_matches_fp64_gpu_at_100k(ctx):
     assert rel_err(got, want) < 5e-6
 
 
+@pytest.mark.parametrize("h,side", [(2.0, 3.0), (2.0, 4.0), (1.0, 2.5), (1.0, 6.0)])
+def test_fmm_leaf_side(ctx, oracle, h, side):
+    """Leaves of side leaf_side * h (> 2h): more pairs in the near field,
+    fewer translations; the near sets still hold every pair within 2h (field)
+    and 4h (force), and the force near field sorts its boxes into the ones
+    that may hold pairs within 4h and the far-only rest for this side."""
+    import magforce
+    pos = slab(40, 36, 10, seed=71)
+    m = f32(random_moments(pos.shape[0], seed=72, scale=0.4))
+    base = magforce.Plan(ctx, dev(pos), h, mode=magforce.MODE_FMM)
+    plan = magforce.Plan(ctx, dev(pos), h, mode=magforce.MODE_FMM, leaf_side=side)
+    bi, pi = base.info(), plan.info()
+    print(f"h={h} side={side}: base {bi} plan {pi}")
+    assert pi["leaves"] < bi["leaves"]
+    assert pi["p2p_field_pairs"] > bi["p2p_field_pairs"]
+    ef = rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, h))
+    eF = rel_err(host(plan.force(dev(m))), oracle.force_allpairs(pos, m, h))
+    print(f"h={h} side={side}: field {ef:.3e} force {eF:.3e}")
+    assert ef < 5e-6 and eF < 5e-6
+
+
+def test_fmm_leaf_size_budget(ctx, oracle):
+    """leaf_size (the reference's ncrit role): the leaf side grows in powers
+    of two from 2h while the non-empty leaves hold at most leaf_size particles
+    on average.  On a unit lattice at h = 1 (8 per leaf at side 2h), a budget
+    of 64 gives side 4h (64 per leaf), 8 keeps 2h."""
+    import magforce
+    pos = slab(32, 32, 12, seed=73)
+    m = f32(random_moments(pos.shape[0], seed=74, scale=0.4))
+    p8 = magforce.Plan(ctx, dev(pos), 1.0, mode=magforce.MODE_FMM, leaf_size=8)
+    p64 = magforce.Plan(ctx, dev(pos), 1.0, mode=magforce.MODE_FMM, leaf_size=64)
+    p4h = magforce.Plan(ctx, dev(pos), 1.0, mode=magforce.MODE_FMM, leaf_side=4.0)
+    assert p8.info()["leaves"] == magforce.Plan(ctx, dev(pos), 1.0, mode=magforce.MODE_FMM).info()["leaves"]
+    assert p64.info()["leaves"] == p4h.info()["leaves"]
+    assert pos.shape[0] / p64.info()["leaves"] <= 64
+    ef = rel_err(host(p64.field(dev(m))), oracle.field_allpairs(pos, pos, m, 1.0))
+    assert ef < 5e-6
+
+
+def test_fmm_leaf_side_rejected_below_2h(ctx):
+    import magforce
+    pos = slab(4, 4, 4, seed=75)
+    with pytest.raises(magforce.MagforceError) as e:
+        magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, leaf_side=1.5)
+    assert e.value.code == magforce.MF_ERR_INVALID_ARG
+
+
+def test_autotuner_leaf_side_search(ctx, slab_case):
+    """With leaf sides in the search, every (side, order, precision) is
+    measured and the choice is the fastest within the target."""
+    import magforce
+    from magforce import tune
+    pos, m, field, _ = slab_case
+    best, cands = tune.autotune_fmm(ctx, dev(pos), 2.0, target_rel_err=1e-5, samples=512,
+                                    orders=(8, 10), precisions=(magforce.M2L_BF16X3,),
+                                    leaf_sides=(2.0, 3.0))
+    assert len(cands) == 4 and sorted({c.leaf_side for c in cands}) == [2.0, 3.0]
+    assert best is not None and best.rel_err <= 1e-5
+    assert not [c for c in cands if c.rel_err <= 1e-5 and c.ms_per_product < best.ms_per_product]
+    plan = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=best.order,
+                         m2l_precision=best.m2l_precision, leaf_side=best.leaf_side)
+    assert rel_err(host(plan.field(dev(m))), field) < 1e-5
+
+
 def test_autotuner_picks_fastest_within_target(ctx, slab_case):
     """The autotuner's choice meets the error target on an independent check,
     and no faster candidate also met it."""

@@ -103,6 +103,52 @@ def test_fp64_refinement_reaches_1e12(ctx, oracle):
     assert rel_err(host(res.x), x_ref) < 1e-11
 
 
+@pytest.mark.parametrize("outer", ["bf16x3", "fp32"])
+def test_fmm_refinement_vs_oracle(ctx, oracle, outer):
+    """Refinement on an FMM plan: x and r = b - A x in FP64, the residual from
+    the plan's FMM product (BF16x3 or FP32 translations), the corrections
+    from CG with one-part BF16 translations (about 4e-3 per translation) to
+    1e-3.  It lands on the FMM operator's solution, within the p = 10
+    truncation of the FP64 solution: the same bar as the plain FMM solve."""
+    import magforce
+    import torch
+    from magforce import configs
+    s = configs.lattice8k()
+    pos = s.pos.astype(np.float64)
+    b = np.tile(s.hconst, (s.n, 1))
+    x_ref, _, _ = oracle.cg_solve(pos, s.h, s.kappa, b, rtol=1e-12)
+    F_ref = oracle.force_allpairs(pos, s.kappa * x_ref, s.h)
+    code = {"bf16x3": magforce.M2L_BF16X3, "fp32": magforce.M2L_FP32}[outer]
+    plan = magforce.Plan(ctx, dev(pos), s.h, mode=magforce.MODE_FMM, m2l_precision=code)
+    res = plan.solve_f64(s.kappa, dev(b, torch.float64), rtol=1e-6,
+                         inner_m2l_precision=magforce.M2L_BF16X1, inner_rtol=1e-3)
+    st = res.stats
+    print(f"fmm refinement ({outer}): {st}")
+    assert st["converged"] == 1 and st["refinements"] >= 1
+    assert st["rel_residual_true"] <= 1e-6
+    x = host(res.x)
+    assert rel_err(x, x_ref) < 5e-6
+    F = host(plan.force(dev(s.kappa * x)))
+    assert rel_err(F, F_ref) < 1e-5
+
+
+def test_bf16x1_product_accuracy(ctx, oracle):
+    """The one-part BF16 translations alone: a few 1e-3 of the far field, so
+    the product stays within 1e-2 of the oracle (a cheap, inexact operator for
+    inner corrections only)."""
+    import magforce
+    from magforce import configs
+    s = configs.lattice8k()
+    pos = s.pos.astype(np.float64)
+    rng = np.random.default_rng(5)
+    m = f32(rng.normal(size=pos.shape) * 0.4)
+    plan = magforce.Plan(ctx, dev(pos), s.h, mode=magforce.MODE_FMM,
+                         m2l_precision=magforce.M2L_BF16X1)
+    err = rel_err(host(plan.field(dev(m))), oracle.field_allpairs(pos, pos, m, s.h))
+    print(f"bf16x1 field error {err:.3e}")
+    assert 1e-6 < err < 1e-2
+
+
 def test_empty_system(ctx):
     import magforce
     import torch
