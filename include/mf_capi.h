@@ -331,6 +331,27 @@ typedef enum mf_dtype {
 int mf_vec_convert(mf_ctx *ctx, int64_t n, int src_dtype, const void *src, int dst_dtype,
                    void *dst);
 
+/* ---- allocation-free variant ------------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the same product as mf_field_allpairs, with every
+ * scratch buffer (packed source and target records, tile boxes, source-chunk
+ * partials) in one static device array of mf_noalloc_scratch_bytes() bytes
+ * (32 MiB) per GPU, held from library load until the process ends.  The call
+ * itself allocates nothing; the context must exist already (mf_ctx_create
+ * looks up the array's address on its device).  Limit: n_t and n_s such that
+ * 32 (pad64(n_s) + 1) + 32 (pad64(n_s) / 64 + 1) + 32 (pad64(n_t) + 1)
+ * + 24 n_t bytes (each term rounded up to 256) fit in the array, e.g.
+ * n_t = n_s = 262144; larger sizes return MF_ERR_INVALID_ARG.  Every call on
+ * one GPU shares the array: calls that overlap on one GPU race. */
+int mf_field_allpairs_noalloc(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                              const float *tz, int64_t n_s, const float *sx, const float *sy,
+                              const float *sz, const float *mx, const float *my,
+                              const float *mz, float h, const float *hconst, float *ox,
+                              float *oy, float *oz);
+/* Size in bytes of that array. */
+int mf_noalloc_scratch_bytes(int64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

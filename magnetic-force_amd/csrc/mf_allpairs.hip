@@ -706,35 +706,71 @@ int64_t choose_chunks(int64_t target_waves, int64_t ntiles, int num_cu) {
 }
 }  // namespace
 
-Status field_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int tstride, int64_t nt,
-                             const SrcRec *src, const float4 *boxes, int64_t ns,
-                             const FieldConsts &fc, const float *hconst, float *ox, float *oy,
-                             float *oz) {
-  if (nt == 0) return Status::Ok();
-  const int64_t ntiles = padded_sources(ns) / kTile;
+namespace {
+struct FieldSplit {
+  int64_t ntiles, nchunk, per;
+  unsigned gx;
+};
+
+// Launch shape of the field kernel: source chunks along grid.y, at most
+// max_chunks of them.
+FieldSplit field_split(mf_ctx *ctx, int64_t nt, int64_t ns, int64_t max_chunks) {
+  FieldSplit f{};
+  f.ntiles = padded_sources(ns) / kTile;
   const int64_t waves = (nt + 64 * kFieldTPT - 1) / (64 * kFieldTPT);
-  const unsigned gx = static_cast<unsigned>((waves + 3) / 4);
-  int64_t nchunk = choose_chunks(waves, ntiles, ctx->num_cu);
-  const int64_t per = ntiles > 0 ? (ntiles + nchunk - 1) / nchunk : 0;
-  nchunk = per > 0 ? (ntiles + per - 1) / per : 1;
-  void *pw = nullptr;
-  MF_TRY(workspace(ctx, "field_partial", sizeof(double) * 3 * nt * nchunk, &pw));
-  double *partial = static_cast<double *>(pw);
-  if (ntiles == 0) {
-    MF_HIP(hipMemsetAsync(partial, 0, sizeof(double) * 3 * nt, ctx->stream));
+  f.gx = static_cast<unsigned>((waves + 3) / 4);
+  int64_t nchunk = std::min(choose_chunks(waves, f.ntiles, ctx->num_cu), std::max<int64_t>(max_chunks, 1));
+  f.per = f.ntiles > 0 ? (f.ntiles + nchunk - 1) / nchunk : 0;
+  f.nchunk = f.per > 0 ? (f.ntiles + f.per - 1) / f.per : 1;
+  return f;
+}
+
+__global__ void zero_f64_kernel(int64_t n, double *__restrict__ a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
+}
+}  // namespace
+
+int64_t field_partial_doubles(int64_t nt, int64_t nchunk) { return 3 * nt * nchunk; }
+
+Status field_allpairs_launch_ws(mf_ctx *ctx, const float4 *tpos, int tstride, int64_t nt,
+                                const SrcRec *src, const float4 *boxes, int64_t ns,
+                                const FieldConsts &fc, const float *hconst, float *ox, float *oy,
+                                float *oz, double *partial, int64_t partial_cap) {
+  if (nt == 0) return Status::Ok();
+  MF_ARG(partial_cap >= field_partial_doubles(nt, 1), "field partial buffer too small");
+  const FieldSplit f = field_split(ctx, nt, ns, partial_cap / field_partial_doubles(nt, 1));
+  if (f.ntiles == 0) {
+    // a fill kernel, not hipMemsetAsync: callers may capture this in a graph
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(grid_for(3 * nt, 256)), dim3(256), 0, ctx->stream,
+                       3 * nt, partial);
+    MF_LAUNCH_CHECK();
   } else {
     KernelTimer timer(ctx, "field");
-    hipLaunchKernelGGL(field_allpairs_kernel<kFieldTPT>, dim3(gx, static_cast<unsigned>(nchunk)),
-                       dim3(kBlock), 0, ctx->stream, tpos, tstride, nt, src, boxes, ntiles, per,
-                       fc, nt, partial);
+    hipLaunchKernelGGL(field_allpairs_kernel<kFieldTPT>, dim3(f.gx, static_cast<unsigned>(f.nchunk)),
+                       dim3(kBlock), 0, ctx->stream, tpos, tstride, nt, src, boxes, f.ntiles,
+                       f.per, fc, nt, partial);
     MF_LAUNCH_CHECK();
   }
   const float h0 = hconst ? hconst[0] : 0.f, h1 = hconst ? hconst[1] : 0.f,
               h2 = hconst ? hconst[2] : 0.f;
   hipLaunchKernelGGL(reduce_field_kernel, dim3(grid_for(nt, 256)), dim3(256), 0, ctx->stream, nt,
-                     nt, static_cast<int>(nchunk), partial, h0, h1, h2, ox, oy, oz);
+                     nt, static_cast<int>(f.nchunk), partial, h0, h1, h2, ox, oy, oz);
   MF_LAUNCH_CHECK();
   return Status::Ok();
+}
+
+Status field_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int tstride, int64_t nt,
+                             const SrcRec *src, const float4 *boxes, int64_t ns,
+                             const FieldConsts &fc, const float *hconst, float *ox, float *oy,
+                             float *oz) {
+  if (nt == 0) return Status::Ok();
+  const FieldSplit f = field_split(ctx, nt, ns, 64);
+  const int64_t cap = field_partial_doubles(nt, f.nchunk);
+  void *pw = nullptr;
+  MF_TRY(workspace(ctx, "field_partial", sizeof(double) * cap, &pw));
+  return field_allpairs_launch_ws(ctx, tpos, tstride, nt, src, boxes, ns, fc, hconst, ox, oy, oz,
+                                  static_cast<double *>(pw), cap);
 }
 
 Status force_allpairs_launch(mf_ctx *ctx, const SrcRec *rec, const float4 *boxes, int64_t n,

@@ -33,6 +33,14 @@ Status field_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int tstride, int64
                              const SrcRec *src, const float4 *boxes, int64_t ns,
                              const FieldConsts &fc, const float *hconst, float *ox, float *oy,
                              float *oz);
+// The same with the chunk partials in caller memory (partial_cap doubles, at
+// least field_partial_doubles(nt, 1)); the source chunk count is cut to fit.
+// Allocates nothing.
+Status field_allpairs_launch_ws(mf_ctx *ctx, const float4 *tpos, int tstride, int64_t nt,
+                                const SrcRec *src, const float4 *boxes, int64_t ns,
+                                const FieldConsts &fc, const float *hconst, float *ox, float *oy,
+                                float *oz, double *partial, int64_t partial_cap);
+int64_t field_partial_doubles(int64_t nt, int64_t nchunk);
 // F_i = (sum_j B(x_i - x_j)[m_j] + Text_i) m_i over one record set.
 Status force_allpairs_launch(mf_ctx *ctx, const SrcRec *rec, const float4 *boxes, int64_t n,
                              const HessConsts &hc, const float *const *text9, float *fx,

@@ -193,6 +193,8 @@ int mf_ctx_create(int device, mf_ctx **out) {
     return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "device index out of range"));
   auto run = [&]() -> Status {
     MF_HIP(hipSetDevice(device));
+    // address of the no-alloc entry point's scratch on this device
+    MF_TRY(mf::noalloc_setup(device));
     mf_ctx *ctx = new (std::nothrow) mf_ctx();
     if (!ctx) return Status::Err(MF_ERR_ALLOC, "out of host memory");
     ctx->device = device;

@@ -194,6 +194,10 @@ class NearOverlap {
   bool forked_ = false, recorded_ = false, joined_ = false;
 };
 
+// Looks up, once per device, the device address of the scratch array of
+// mf_field_allpairs_noalloc (mf_noalloc.hip); `device` must be current.
+Status noalloc_setup(int device);
+
 // Collects finished event pairs into the totals (synchronizes on them).
 Status flush_timers(mf_ctx *ctx);
 

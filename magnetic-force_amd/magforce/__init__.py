@@ -142,6 +142,9 @@ _SIGS = {
     "mf_vec_mul": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     "mf_vec_dot": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_d)], _i),
     "mf_vec_fill": ([_vp, _i64, _f, _vp], _i),
+    "mf_field_allpairs_noalloc": ([_vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _f, _vp, _vp, _vp, _vp], _i),
+    "mf_noalloc_scratch_bytes": ([ctypes.POINTER(_i64)], _i),
 }
 
 
@@ -254,6 +257,32 @@ def field_allpairs(ctx: Context, tpos, spos, smom, h: float, hconst=None):
                                    _ptr(m, 1), _ptr(m, 2), float(h), hc, _ptr(out, 0),
                                    _ptr(out, 1), _ptr(out, 2)), ctx.h)
     return out
+
+
+def field_allpairs_noalloc(ctx: Context, tpos, spos, smom, h: float, hconst=None, out=None):
+    """field_allpairs through the allocation-free entry point: its scratch is a
+    static device array of noalloc_scratch_bytes() per GPU.  `out` may be a
+    preallocated (3, n_t) float32 tensor."""
+    import torch
+    t = _soa(tpos)
+    s = _soa(spos)
+    m = _soa(smom, s.shape[1])
+    if out is None:
+        out = torch.empty_like(t)
+    hc = None
+    if hconst is not None:
+        hc = (ctypes.c_float * 3)(*[float(v) for v in hconst])
+    _check(lib().mf_field_allpairs_noalloc(ctx.h, t.shape[1], _ptr(t, 0), _ptr(t, 1), _ptr(t, 2),
+                                           s.shape[1], _ptr(s, 0), _ptr(s, 1), _ptr(s, 2),
+                                           _ptr(m, 0), _ptr(m, 1), _ptr(m, 2), float(h), hc,
+                                           _ptr(out, 0), _ptr(out, 1), _ptr(out, 2)), ctx.h)
+    return out
+
+
+def noalloc_scratch_bytes() -> int:
+    v = ctypes.c_int64()
+    _check(lib().mf_noalloc_scratch_bytes(ctypes.byref(v)))
+    return v.value
 
 
 def field_allpairs_f64(ctx: Context, tpos, spos, smom, h: float, hconst=None):
