@@ -16,7 +16,10 @@ Printed JSON line (rank 0): metric/value/unit per the contract, plus
   "timestep"    : the moving-particle step (plan rebuilt in place, solve
                   warm-started from the last solution): ms/step, rebuild
                   time and CG iterations warm against cold,
-  "cpu_baseline": the CPU oracle timed on a bounded sample of the workload.
+  "refined_fmm" : the step with FP64-residual refinement on the FMM plan
+                  (inner corrections with one-part BF16 translations),
+  "cpu_baseline": the reference's CPU FMM path restated in FP64 (oracle/), one
+                  field product of the workload timed on the host cores.
 """
 from __future__ import annotations
 
@@ -186,6 +189,58 @@ def time_fp32_fmm(ctx, system, args):
            "solver": st.stats}
     st.plan.close()
     log(f"fp32 fmm: {out}")
+    return out
+
+
+def time_refined(ctx, system, args, m2l_code, samples=4096):
+    """The same step with the magnetization solved by iterative refinement on
+    the FMM plan (mf_plan_solve_f64): x and r = b - A x in FP64, the residual
+    from the plan's own FMM product (M2L at the step's precision), the inner
+    corrections by CG with one-part BF16 translations to 1e-3.  Reported beside
+    the plain mixed-precision CG step, with its own error against FP64."""
+    import torch
+    import magforce
+    s = system
+    pos = to_dev(s.pos)
+    b64 = magforce.uniform_field(s.n, s.hconst).double()
+    F = torch.empty((3, s.n), dtype=torch.float32, device="cuda")
+    state = {}
+
+    def step():
+        if "plan" not in state:
+            state["plan"] = magforce.Plan(ctx, pos, s.h, mode=magforce.MODE_FMM,
+                                          m2l_precision=m2l_code)
+        else:
+            state["plan"].update_positions(pos)
+        res = state["plan"].solve_f64(s.kappa, b64, rtol=args.rtol, max_refine=8,
+                                      inner_m2l_precision=magforce.M2L_BF16X1, inner_rtol=1e-3)
+        state["res"] = res
+        state["plan"].force((s.kappa * res.x).float(), out=F)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    out = {"ms_per_step": e0.elapsed_time(e1) / args.steps, "outer_m2l": args.m2l,
+           "inner_m2l": "bf16x1", "inner_rtol": 1e-3, "solver": state["res"].stats}
+    if not args.no_check:
+        ap = magforce.Plan(ctx, pos, s.h)
+        ref = ap.solve_f64(s.kappa, b64, rtol=1e-12, max_refine=6)
+        rng = np.random.default_rng(12345)
+        idx = torch.tensor(np.sort(rng.choice(s.n, size=min(samples, s.n), replace=False)),
+                           dtype=torch.int64, device="cuda")
+        F_ref = magforce.force_subset_f64(ctx, pos.double(), s.kappa * ref.x, s.h, idx)
+        out["force_rel_l2"] = float(torch.linalg.norm(F[:, idx].double() - F_ref) /
+                                    torch.linalg.norm(F_ref))
+        out["x_rel_l2"] = float(torch.linalg.norm(state["res"].x - ref.x) / torch.linalg.norm(ref.x))
+        ap.close()
+    state["plan"].close()
+    log(f"refined fmm: {out}")
     return out
 
 
@@ -454,39 +509,67 @@ def run_c1(ctx):
     return out
 
 
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline(system, budget_s, matvecs_per_step):
-    """The CPU oracle (FP64 all-pairs restatement) on a bounded sample of
-    target rows, extrapolated to one solve + force step of the workload."""
+    """The reference's CPU path for the step, restated in FP64 (oracle/mf_cpufmm.c):
+    its fmmtl FMM field product (ncrit-split octree with ncrit 384, dual
+    traversal with theta 0.5 and the 3h gap, P = 6 terms, smoothed near
+    pairs), timed for one product of the whole workload on the host's cores.
+    The step is that product times (matvecs + 2): the solve's products plus a
+    force product costed at two field products (its pair work is twice the
+    field's).  A bounded FP64 all-pairs row sample, extrapolated, is reported
+    beside it as a second figure."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(16, os.cpu_count() or 1)
+    threads = min(16, os.cpu_count() or 1)  # the box's CPU share for one GPU
     pos = system.pos.astype(np.float64)
     n = system.n
     rng = np.random.default_rng(1)
     m = rng.normal(size=(n, 3)) * 0.5
-    rows = 64
-    t_field = t_force = 0.0
-    done_f = done_h = 0
+    t0 = time.perf_counter()
+    field, st = oracle.fmm_field(pos, m, system.h, ncrit=384, theta=0.5, hconst=None,
+                                 nthreads=threads)
+    t_fmm = time.perf_counter() - t0
+    # its accuracy on 256 seeded rows against the FP64 all-pairs oracle
+    sel = rng.choice(n, size=min(256, n), replace=False)
+    ref = oracle.field_allpairs(pos[sel], pos, m, system.h, nthreads=threads)
+    err = float(np.linalg.norm(field[sel] - ref) / np.linalg.norm(ref))
+    # second figure: FP64 all-pairs rows, extrapolated
+    rows, done, t_ap = 64, 0, 0.0
     t_start = time.perf_counter()
-    while time.perf_counter() - t_start < budget_s * 0.5:
-        sel = rng.choice(n, size=rows, replace=False)
-        t0 = time.perf_counter()
-        oracle.field_allpairs(pos[sel], pos, m, system.h, nthreads=threads)
-        t_field += time.perf_counter() - t0
-        done_f += rows
-    while time.perf_counter() - t_start < budget_s:
-        sel = rng.choice(n, size=rows // 4, replace=False)
-        t0 = time.perf_counter()
-        oracle.hessian_allpairs(pos[sel], pos, m, system.h, nthreads=threads)
-        t_force += time.perf_counter() - t0
-        done_h += rows // 4
-    per_field = t_field / done_f * n          # s per full field product
-    per_force = t_force / max(done_h, 1) * n  # s per full force product
-    step_ms = (matvecs_per_step * per_field + per_force) * 1e3
-    return {"value": step_ms, "unit": "ms/step", "cores": threads, "kind": "port",
-            "sample": (f"FP64 all-pairs oracle rows: {done_f} field and {done_h} force-tensor "
-                       f"target rows x {n} sources ({t_field + t_force:.1f} s), extrapolated to "
-                       f"{matvecs_per_step} field products + 1 force product")}
+    while time.perf_counter() - t_start < budget_s * 0.25:
+        s_rows = rng.choice(n, size=rows, replace=False)
+        t1 = time.perf_counter()
+        oracle.field_allpairs(pos[s_rows], pos, m, system.h, nthreads=threads)
+        t_ap += time.perf_counter() - t1
+        done += rows
+    ap_step_ms = t_ap / max(done, 1) * n * (matvecs_per_step + 2) * 1e3
+    out = {"value": t_fmm * (matvecs_per_step + 2) * 1e3, "unit": "ms/step", "cores": threads,
+           "kind": "port",
+           "sample": (f"one FP64 CPU FMM field product of the whole workload (restatement of the "
+                      f"reference's fmmtl path: ncrit 384, theta 0.5, 3h gap, P = 6; "
+                      f"{st['leaves']} leaves, {st['far_box_pairs']} far and "
+                      f"{st['near_box_pairs']} near box pairs) took {t_fmm:.1f} s on {threads} "
+                      f"threads; step = {matvecs_per_step} field products + a force product "
+                      f"costed as 2"),
+           "fmm_field_product_s": t_fmm, "fmm_rel_err_vs_fp64_allpairs": err, "fmm_tree": st,
+           "allpairs_extrapolation": {"value": ap_step_ms, "unit": "ms/step",
+                                      "sample": f"{done} FP64 all-pairs target rows x {n} "
+                                                f"sources in {t_ap:.1f} s, extrapolated"}}
+    out.update(cpu_info())
+    return out
 
 
 def main():
@@ -652,6 +735,9 @@ def main():
     if mode == magforce.MODE_FMM and args.m2l != "fp32" and not args.no_compare:
         compare = time_fp32_fmm(ctx, system, args)
         compare["speedup"] = compare["ms_per_step"] / ms_step
+    refined = None
+    if mode == magforce.MODE_FMM and not args.no_compare and n <= 2_000_000:
+        refined = time_refined(ctx, system, args, m2l_code)
     batch = None if args.no_batch else run_droplets(ctx, args, m2l_code)
     c1 = None if args.no_cpu else run_c1(ctx)
     allpairs = None if args.no_allpairs else measure_allpairs(ctx)
@@ -685,6 +771,7 @@ def main():
                    "solver": st.stats},
         "error": check,
         "fp32_fmm": compare,
+        "refined_fmm": refined,
         "batch": batch,
         "c1_lattice8k": c1,
         "allpairs": allpairs,

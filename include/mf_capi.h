@@ -133,6 +133,16 @@ int mf_hessian_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *
                         const float *sz, const float *mx, const float *my, const float *mz,
                         float h, float *out6);
 
+/* The same tensor written as the nine arrays the force takes: out9 is a host
+ * array of 9 device pointers in the order of text9 below (the reference's
+ * dHextdx.{x,y,z}, dHextdy.{x,y,z}, dHextdz.{x,y,z}, which
+ * cuda_ext_fitted_force_tensor_kernel fills, magnetic_force.cu:432-512 and
+ * :1164-1172), each symmetric component written to both of its places. */
+int mf_hessian_allpairs9(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                         const float *tz, int64_t n_s, const float *sx, const float *sy,
+                         const float *sz, const float *mx, const float *my, const float *mz,
+                         float h, float *const *out9);
+
 /* Force on one particle set: F_i = (sum_j B(x_i - x_j)[m_j] + Text_i) m_i.
  * Replaces fitted_force_ofM_kernel (magnetic_force.cu:516-605).  text9 is a
  * host array of 9 device pointers (the reference's dHextdx.{x,y,z},

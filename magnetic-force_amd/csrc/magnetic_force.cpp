@@ -247,32 +247,16 @@ void cuda_magnet_cgOptimizer::eval_hext() {
                                   Hext_source_M.dev_x, Hext_source_M.dev_y, Hext_source_M.dev_z,
                                   ext_mag_h, hc, Hext.dev_x, Hext.dev_y, Hext.dev_z),
                 ctx);
-  // 6 symmetric components xx yy zz xy xz yz -> dHextd{x,y,z} columns
-  float *t6 = nullptr;
-  detail::check_hip(hipMalloc(reinterpret_cast<void **>(&t6), 6 * np * sizeof(float)), "eval_hext");
-  const int rc = mf_hessian_allpairs(ctx, static_cast<int64_t>(np), xyz.dev_x, xyz.dev_y,
+  // the nine dHextd{x,y,z} arrays straight from the tensor kernel
+  float *const out9[9] = {dHextdx.dev_x, dHextdx.dev_y, dHextdx.dev_z,
+                          dHextdy.dev_x, dHextdy.dev_y, dHextdy.dev_z,
+                          dHextdz.dev_x, dHextdz.dev_y, dHextdz.dev_z};
+  detail::check(mf_hessian_allpairs9(ctx, static_cast<int64_t>(np), xyz.dev_x, xyz.dev_y,
                                      xyz.dev_z, static_cast<int64_t>(next), Hext_source_pos.dev_x,
                                      Hext_source_pos.dev_y, Hext_source_pos.dev_z,
                                      Hext_source_M.dev_x, Hext_source_M.dev_y,
-                                     Hext_source_M.dev_z, ext_mag_h, t6);
-  if (rc != MF_OK) {
-    (void)hipFree(t6);
-    detail::check(rc, ctx);
-  }
-  const size_t B = np * sizeof(float);
-  float *xx = t6, *yy = t6 + np, *zz = t6 + 2 * np, *xy = t6 + 3 * np, *xz = t6 + 4 * np,
-        *yz = t6 + 5 * np;
-  const float *src[9] = {xx, xy, xz, xy, yy, yz, xz, yz, zz};
-  float *dst[9] = {dHextdx.dev_x, dHextdx.dev_y, dHextdx.dev_z, dHextdy.dev_x, dHextdy.dev_y,
-                   dHextdy.dev_z, dHextdz.dev_x, dHextdz.dev_y, dHextdz.dev_z};
-  for (int k = 0; k < 9; ++k) {
-    const hipError_t e = hipMemcpy(dst[k], src[k], B, hipMemcpyDeviceToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(t6);
-      detail::check_hip(e, "eval_hext");
-    }
-  }
-  (void)hipFree(t6);
+                                     Hext_source_M.dev_z, ext_mag_h, out9),
+                ctx);
 }
 
 // magnetic_force.cu:1196-1271

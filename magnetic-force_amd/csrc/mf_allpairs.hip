@@ -413,9 +413,15 @@ __global__ __launch_bounds__(kBlock) void force_allpairs_kernel(
 // Force tensor (6 components).  One target per lane; mixed loop only (this is
 // the external-source product, not on the solve path).
 
+// Force-tensor field at separate targets (the external tensor T_ext of
+// eval_hext), tiled like the field kernel: one target per lane, sources in
+// 64-record tiles through scalar loads; a tile whose box lies beyond 4h of the
+// wave's targets runs the branch-free dipole form.  Chunks of source tiles
+// along grid.y write FP64 partials [chunk][6][nt_ld] (xx yy zz xy xz yz).
 __global__ __launch_bounds__(kBlock) void hessian_allpairs_kernel(
-    const float4 *__restrict__ tpos, int64_t nt, const SrcRec *__restrict__ src, int64_t ns_pad,
-    HessConsts hc, float *__restrict__ out6) {
+    const float4 *__restrict__ tpos, int64_t nt, const SrcRec *__restrict__ src,
+    const float4 *__restrict__ boxes, int64_t ntiles, int64_t tiles_per_chunk, HessConsts hc,
+    int64_t nt_ld, double *__restrict__ partial) {
   __shared__ float4 s_tab[12];
   if (threadIdx.x < 12) {
     const float *row = &hc.curve[threadIdx.x / 3][(threadIdx.x % 3) * 4];
@@ -423,15 +429,32 @@ __global__ __launch_bounds__(kBlock) void hessian_allpairs_kernel(
   }
   __syncthreads();
   const float *tab = reinterpret_cast<const float *>(s_tab);
-  int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  int64_t i = wave * 64 + lane;
+  if (wave * 64 >= nt) return;  // whole wave idle (uniform)
   const bool valid = i < nt;
   i = valid ? i : nt - 1;
   const float4 t = tpos[i];
+  float lx = valid ? t.x : INFINITY, ly = valid ? t.y : INFINITY, lz = valid ? t.z : INFINITY;
+  float hx = valid ? t.x : -INFINITY, hy = valid ? t.y : -INFINITY, hz = valid ? t.z : -INFINITY;
+  lx = uniform_f(wave_min(lx));
+  ly = uniform_f(wave_min(ly));
+  lz = uniform_f(wave_min(lz));
+  hx = uniform_f(wave_max(hx));
+  hy = uniform_f(wave_max(hy));
+  hz = uniform_f(wave_max(hz));
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t tile = 0; tile < ns_pad; tile += kTile) {
+  const int64_t chunk = blockIdx.y;
+  const int64_t t0 = chunk * tiles_per_chunk;
+  const int64_t t1 = t0 + tiles_per_chunk < ntiles ? t0 + tiles_per_chunk : ntiles;
+  for (int64_t tile = t0; tile < t1; ++tile) {
+    const float4 slo = boxes[2 * tile];
+    const float4 shi = boxes[2 * tile + 1];
+    const float dist2 = uniform_f(box_dist2(lx, ly, lz, hx, hy, hz, slo, shi));
     float a[6] = {0, 0, 0, 0, 0, 0};
-    const SrcRec *__restrict__ s = src + tile;
-    for (int j = 0; j < kTile; ++j) {
+    const SrcRec *__restrict__ s = src + tile * kTile;
+    auto pair = [&](int j, bool near) {
       const float dx = t.x - s[j].x, dy = t.y - s[j].y, dz = t.z - s[j].z;
       const float mx = s[j].mx, my = s[j].my, mz = s[j].mz;
       const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -439,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void hessian_allpairs_kernel(
       const float ir2 = ir * ir;
       float P = ir2 * ir2 * ir;
       float R = -5.f * P * ir2;
-      if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, P, R);
+      if (near && !(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, P, R);
       const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
       const float Rdm = R * dm;
       const float Pdm = P * dm;
@@ -449,15 +472,45 @@ __global__ __launch_bounds__(kBlock) void hessian_allpairs_kernel(
       a[3] = fmaf(P, fmaf(dx, my, mx * dy), fmaf(Rdm * dx, dy, a[3]));
       a[4] = fmaf(P, fmaf(dx, mz, mx * dz), fmaf(Rdm * dx, dz, a[4]));
       a[5] = fmaf(P, fmaf(dy, mz, my * dz), fmaf(Rdm * dy, dz, a[5]));
+    };
+    if (dist2 > hc.near2) {
+#pragma unroll 4
+      for (int j = 0; j < kTile; ++j) pair(j, false);
+    } else {
+#pragma unroll 2
+      for (int j = 0; j < kTile; ++j) pair(j, true);
     }
 #pragma unroll
     for (int c = 0; c < 6; ++c) acc[c] += static_cast<double>(a[c]);
   }
   if (valid) {
-    const double sfac = 3.0 * kMu0 / (4.0 * kPi);
+    double *out = partial + chunk * 6 * nt_ld;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) out6[c * nt + i] = static_cast<float>(sfac * acc[c]);
+    for (int c = 0; c < 6; ++c) out[c * nt_ld + i] = acc[c];
   }
+}
+
+// Destinations of the six tensor components: out[k] (may be null) receives
+// component sym[k] (0..5 = xx yy zz xy xz yz) times mu0 3/(4 pi).
+struct TensorOut {
+  float *p[9];
+  int sym[9];
+};
+
+__global__ void reduce_hessian_kernel(int64_t n, int64_t nt_ld, int nchunk,
+                                      const double *__restrict__ partial, TensorOut out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s[6] = {0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < nchunk; ++c) {
+    const double *p = partial + static_cast<int64_t>(c) * 6 * nt_ld;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] += p[k * nt_ld + i];
+  }
+  const double sfac = 3.0 * kMu0 / (4.0 * kPi);
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+    if (out.p[k]) out.p[k][i] = static_cast<float>(sfac * s[out.sym[k]]);
 }
 
 // ---------------------------------------------------------------------------
@@ -809,11 +862,36 @@ Status force_allpairs_launch(mf_ctx *ctx, const SrcRec *rec, const float4 *boxes
 }
 
 Status hessian_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int64_t nt, const SrcRec *src,
-                               int64_t ns, const HessConsts &hc, float *out6) {
+                               const float4 *boxes, int64_t ns, const HessConsts &hc,
+                               float *const dst[9], const int sym[9]) {
   if (nt == 0) return Status::Ok();
-  KernelTimer timer(ctx, "hessian");
-  hipLaunchKernelGGL(hessian_allpairs_kernel, dim3(grid_for(nt, kBlock)), dim3(kBlock), 0,
-                     ctx->stream, tpos, nt, src, padded_sources(ns), hc, out6);
+  const int64_t ntiles = padded_sources(ns) / kTile;
+  const int64_t waves = (nt + 63) / 64;
+  const unsigned gx = static_cast<unsigned>((waves + 3) / 4);
+  int64_t nchunk = choose_chunks(waves, ntiles, ctx->num_cu);
+  const int64_t per = ntiles > 0 ? (ntiles + nchunk - 1) / nchunk : 0;
+  nchunk = per > 0 ? (ntiles + per - 1) / per : 1;
+  void *pw = nullptr;
+  MF_TRY(workspace(ctx, "hessian_partial", sizeof(double) * 6 * nt * nchunk, &pw));
+  double *partial = static_cast<double *>(pw);
+  if (ntiles == 0) {
+    hipLaunchKernelGGL(zero_f64_kernel, dim3(grid_for(6 * nt, 256)), dim3(256), 0, ctx->stream,
+                       6 * nt, partial);
+    MF_LAUNCH_CHECK();
+  } else {
+    KernelTimer timer(ctx, "hessian");
+    hipLaunchKernelGGL(hessian_allpairs_kernel, dim3(gx, static_cast<unsigned>(nchunk)),
+                       dim3(kBlock), 0, ctx->stream, tpos, nt, src, boxes, ntiles, per, hc, nt,
+                       partial);
+    MF_LAUNCH_CHECK();
+  }
+  TensorOut out{};
+  for (int k = 0; k < 9; ++k) {
+    out.p[k] = dst[k];
+    out.sym[k] = sym[k];
+  }
+  hipLaunchKernelGGL(reduce_hessian_kernel, dim3(grid_for(nt, 256)), dim3(256), 0, ctx->stream,
+                     nt, nt, static_cast<int>(nchunk), partial, out);
   MF_LAUNCH_CHECK();
   return Status::Ok();
 }

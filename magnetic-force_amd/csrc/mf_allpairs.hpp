@@ -45,9 +45,12 @@ int64_t field_partial_doubles(int64_t nt, int64_t nchunk);
 Status force_allpairs_launch(mf_ctx *ctx, const SrcRec *rec, const float4 *boxes, int64_t n,
                              const HessConsts &hc, const float *const *text9, float *fx,
                              float *fy, float *fz);
-// 6-component force tensor at separate targets.
+// Force tensor at separate targets: dst[k] (null = skipped) receives the
+// symmetric component sym[k] (0..5 = xx yy zz xy xz yz).  boxes: tile boxes
+// of the source records.
 Status hessian_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int64_t nt, const SrcRec *src,
-                               int64_t ns, const HessConsts &hc, float *out6);
+                               const float4 *boxes, int64_t ns, const HessConsts &hc,
+                               float *const dst[9], const int sym[9]);
 
 // FP64 check path (mf_check64.hip).
 Status field_allpairs_f64(mf_ctx *ctx, int64_t nt, const double *tx, const double *ty,

@@ -366,36 +366,71 @@ int mf_field_allpairs_f64(mf_ctx *ctx, int64_t n_t, const double *tx, const doub
   return mf::finish(ctx, run());
 }
 
-int mf_hessian_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
-                        const float *tz, int64_t n_s, const float *sx, const float *sy,
-                        const float *sz, const float *mx, const float *my, const float *mz,
-                        float h, float *out6) {
+namespace {
+// Shared body of mf_hessian_allpairs and mf_hessian_allpairs9.
+int hessian_entry(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty, const float *tz,
+                  int64_t n_s, const float *sx, const float *sy, const float *sz, const float *mx,
+                  const float *my, const float *mz, float h, float *const dst[9],
+                  const int sym[9]) {
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n_t >= 0 && n_s >= 0, "sizes must be >= 0");
     MF_ARG(h > 0.f, "h must be > 0");
-    MF_ARG(n_t == 0 || (tx && ty && tz && out6), "target/output pointers are null");
+    MF_ARG(n_t == 0 || (tx && ty && tz), "target pointers are null");
+    for (int k = 0; k < 9; ++k) MF_ARG(n_t == 0 || sym[k] < 0 || dst[k], "output pointers are null");
     MF_ARG(n_s == 0 || (sx && sy && sz && mx && my && mz), "source pointers are null");
     MF_HIP(hipSetDevice(ctx->device));
     const int64_t np = mf::padded_sources(n_s);
     void *p = nullptr;
     MF_TRY(mf::workspace(ctx, "ext_src", sizeof(mf::SrcRec) * static_cast<size_t>(np + 1), &p));
     mf::SrcRec *src = static_cast<mf::SrcRec *>(p);
+    MF_TRY(mf::workspace(ctx, "ext_boxes", sizeof(float4) * 2 * static_cast<size_t>(np / mf::kTile + 1), &p));
+    float4 *boxes = static_cast<float4 *>(p);
     MF_TRY(mf::workspace(ctx, "ext_tgt",
                          sizeof(mf::SrcRec) * static_cast<size_t>(mf::padded_sources(n_t) + 1), &p));
     mf::SrcRec *tgt = static_cast<mf::SrcRec *>(p);
     MF_TRY(mf::pack_sources(ctx, n_s, sx, sy, sz, mx, my, mz, 1.f, src));
+    MF_TRY(mf::tile_boxes(ctx, n_s, src, boxes));
     if (n_t > 0) MF_TRY(mf::pack_sources(ctx, n_t, tx, ty, tz, nullptr, nullptr, nullptr, 0.f, tgt));
-    // hessian kernel reads float4 targets with unit stride: use the packed
-    // records' first half through a compacting view.
+    // the kernel reads the records' position halves with a stride of two float4
     MF_TRY(mf::workspace(ctx, "ext_tpos", sizeof(float4) * static_cast<size_t>(n_t + 1), &p));
     float4 *tpos = static_cast<float4 *>(p);
     if (n_t > 0)
       MF_HIP(hipMemcpy2DAsync(tpos, sizeof(float4), tgt, sizeof(mf::SrcRec), sizeof(float4), n_t,
                               hipMemcpyDeviceToDevice, ctx->stream));
-    return mf::hessian_allpairs_launch(ctx, tpos, n_t, src, n_s, mf::make_hess_consts(h), out6);
+    float *d[9];
+    int sy9[9];
+    for (int k = 0; k < 9; ++k) {
+      d[k] = sym[k] < 0 ? nullptr : dst[k];
+      sy9[k] = sym[k] < 0 ? 0 : sym[k];
+    }
+    return mf::hessian_allpairs_launch(ctx, tpos, n_t, src, boxes, n_s, mf::make_hess_consts(h), d,
+                                       sy9);
   };
   return mf::finish(ctx, run());
+}
+}  // namespace
+
+int mf_hessian_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                        const float *tz, int64_t n_s, const float *sx, const float *sy,
+                        const float *sz, const float *mx, const float *my, const float *mz,
+                        float h, float *out6) {
+  float *dst[9] = {};
+  const int sym[9] = {0, 1, 2, 3, 4, 5, -1, -1, -1};
+  for (int k = 0; k < 6; ++k) dst[k] = out6 ? out6 + static_cast<int64_t>(k) * n_t : nullptr;
+  return hessian_entry(ctx, n_t, tx, ty, tz, n_s, sx, sy, sz, mx, my, mz, h, dst, sym);
+}
+
+int mf_hessian_allpairs9(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty,
+                         const float *tz, int64_t n_s, const float *sx, const float *sy,
+                         const float *sz, const float *mx, const float *my, const float *mz,
+                         float h, float *const *out9) {
+  if (!out9) return mf::finish(ctx, Status::Err(MF_ERR_INVALID_ARG, "out9 is null"));
+  // row a, column b of T -> symmetric component: xx xy xz / xy yy yz / xz yz zz
+  const int sym[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+  float *dst[9];
+  for (int k = 0; k < 9; ++k) dst[k] = out9[k];
+  return hessian_entry(ctx, n_t, tx, ty, tz, n_s, sx, sy, sz, mx, my, mz, h, dst, sym);
 }
 
 int mf_force_allpairs(mf_ctx *ctx, int64_t n, const float *x, const float *y, const float *z,

@@ -17,8 +17,10 @@
 //    hereditary, worst M2L separation ratio 0.61.  The field near field is N
 //    at the leaf level (covers every pair with q < 2); the force near field is
 //    NH = {o : sum max(|o_i| - 1, 0)^2 <= 4} (179 boxes: every pair within 4h,
-//    where the reference's fitted curves live).  Interaction lists: children
-//    of the parent's N-neighbours minus the near set.
+//    where the reference's fitted curves live, and the boxes exactly two leaf
+//    sides away, whose pairs take the dipole form: translating those instead
+//    costs the force twice its error for 13 % of its time).  Interaction
+//    lists: children of the parent's N-neighbours minus the near set.
 //  * Expansions in scaled real dofs (mf_expansion.hpp): M2M/L2L are 8 fixed
 //    matrices, M2L one fixed matrix per offset (column-major tables in HBM).
 //  * M2L runs on MFMA (FP32, or BF16 with split operands): a workgroup owns
@@ -38,6 +40,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <type_traits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -59,10 +62,10 @@ namespace {
 
 constexpr int kNearF = 81;
 constexpr int kNearFMixed = 27;  // slots with |o|_inf <= 1 may hold q < 2 pairs
-// Force near set: the boxes less than two leaf sides away (every pair within
+// Force near set: the boxes at most two leaf sides apart (every pair within
 // 4h for leaf sides >= 2h).  Its slots are ordered by that separation; the
-// leading ones that may hold pairs within 4h (at leaf side 2h the 125 boxes
-// of {-2..2}^3) run the near test, the rest (at 2h the 54 boxes exactly two
+// prefix that may hold pairs within 4h (at leaf side 2h the 125 boxes of
+// {-2..2}^3) runs the near test, the rest (at 2h the 54 boxes exactly two
 // leaf sides away) take the far form with no test.
 constexpr int kNearH = 179;
 constexpr int kMaxLevels = 21;
@@ -117,6 +120,19 @@ __device__ inline int find_key(const uint64_t *__restrict__ keys, int count, uin
 // Morton bits (a plan may hold several independent particle systems, each
 // with its own root cube of the common side L); lo[3 sys + a] is that
 // system's root corner.
+// Products of a batch plan inside its batched solve skip the boxes of
+// systems that have converged: box_sys[box] is the box's system, active[s]
+// (device, written by the solver) whether system s still iterates.  active ==
+// nullptr: every box runs.  The skipped boxes keep their last outputs, which
+// the solver no longer reads.
+struct SysMask {
+  const int32_t *box_sys = nullptr;
+  const int32_t *active = nullptr;
+};
+__device__ __forceinline__ bool sys_skip(const SysMask &m, int box) {
+  return m.active != nullptr && m.active[m.box_sys[box]] == 0;
+}
+
 struct Frame {
   const double *__restrict__ lo;
   int shift;  // 3 l
@@ -337,7 +353,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
                                                  const int32_t *__restrict__ count,
                                                  const uint64_t *__restrict__ keys,
                                                  const SrcRec *__restrict__ rec, Frame fr,
-                                                 double s, float *__restrict__ Mt) {
+                                                 double s, float *__restrict__ Mt, SysMask mask) {
   constexpr int ND = (P + 1) * (P + 1);
   // The wave's sums go through LDS in two halves of the dofs: a 64 x ND
   // buffer (31 KB at p = 10) would hold P2M to 5 waves per CU.
@@ -345,6 +361,7 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
   static_assert(NH <= 64, "one dof per lane in each half");
   __shared__ float red[64][NH + 1];
   const int b = blockIdx.x;
+  if (sys_skip(mask, b)) return;  // uniform over the wave, before any barrier
   const int lane = threadIdx.x;
   double cx, cy, cz;
   box_centre(fr, keys[b], s, cx, cy, cz);
@@ -468,18 +485,6 @@ struct M2LGather {
 };
 
 template <int KP, int NPART>
-__device__ __forceinline__ void m2l_src_load(const int *s_tgt, const int32_t *__restrict__ src_slots,
-                                             int src_ld, int slot, int tid,
-                                             int (&src)[M2LGather<KP, NPART>::IT]) {
-  using G = M2LGather<KP, NPART>;
-#pragma unroll
-  for (int it = 0; it < G::IT; ++it) {
-    const int tgt = s_tgt[(tid + it * 2 * KP) / G::PER_T];
-    src[it] = tgt >= 0 ? src_slots[static_cast<int64_t>(tgt) * src_ld + slot] : -1;
-  }
-}
-
-template <int KP, int NPART>
 __device__ __forceinline__ int m2l_mult_load(const void *__restrict__ Msrc,
                                              const int (&src)[M2LGather<KP, NPART>::IT], int tid,
                                              float4 (&v)[M2LGather<KP, NPART>::IT]) {
@@ -515,8 +520,20 @@ __global__ void split_multipoles_kernel(int64_t nelem, int KP, const float *__re
 // OCTANT marks the instances that run the M2M and L2L passes (octant
 // matrices).  The code is the same; the separate instances let kernel traces
 // list those passes apart from the M2L.
+// Waves per SIMD the M2L instances are built for: three (at most 168
+// registers) for the FP32 and one- and two-part kernels, whose operator
+// prefetch would otherwise take the compiler to 196 registers and two waves
+// (measured slower than without the prefetch); the three-part kernel and the
+// p = 8 two-part kernel do not fit three without spilling and keep the
+// compiler's choice.
+template <int KP, int NPART>
+constexpr int m2l_min_waves() {
+  return (NPART == 3 || (NPART == 2 && KP != 128)) ? 1 : 3;
+}
+
 template <int KP, int NPART, bool OCTANT = false>
-__global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Msrc,
+__global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(m2l_min_waves<KP, NPART>())))
+void m2l_kernel(const void *__restrict__ Msrc,
                                                           float *__restrict__ Lt,
                                                           const void *__restrict__ Tv,
                                                           const M2LTile *__restrict__ tiles,
@@ -528,7 +545,8 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
                                                           int slot_ld,
                                                           const int32_t *__restrict__ tile_list,
                                                           const int32_t *__restrict__ tile_nlist,
-                                                          int list_ld, int accumulate) {
+                                                          int list_ld, int accumulate,
+                                                          SysMask mask) {
   constexpr int NT = 2 * KP;
   constexpr int LDF = KP + 4;  // FP32 row pitch (floats)
   // BF16 row pitch (elements).  Pitches of KP + 8, + 16 and + 24 are all
@@ -547,6 +565,12 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
   const int wave = tid >> 6, lane = tid & 63;
   for (int j = tid; j < kTileT; j += NT) s_tgt[j] = j < tile.count ? sorted_boxes[tile.first + j] : -1;
   __syncthreads();
+  if (mask.active) {
+    // a tile whose target boxes all belong to converged systems is skipped
+    int live = 0;
+    for (int j = tid; j < tile.count; j += NT) live |= mask.active[mask.box_sys[s_tgt[j]]];
+    if (!__syncthreads_or(live)) return;  // uniform over the workgroup
+  }
   typedef float f4 __attribute__((ext_vector_type(4)));
   f4 acc[2][4];
 #pragma unroll
@@ -560,12 +584,54 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
   const int32_t *__restrict__ tl = tile_list + static_cast<int64_t>(tile_id) * list_ld;
   const int na = tile_nlist[tile_id];
   using GA = M2LGather<KP, NPART>;
+  // the target box of each of this thread's gather chunks: fixed for the tile,
+  // so the source-slot loads of a later offset wait on no LDS read
+  int tgt_of[GA::IT];
+#pragma unroll
+  for (int it = 0; it < GA::IT; ++it) tgt_of[it] = s_tgt[(tid + it * NT) / GA::PER_T];
   int src[GA::IT];
   float4 v[GA::IT];
+  auto src_load = [&](int slot) {
+#pragma unroll
+    for (int it = 0; it < GA::IT; ++it)
+      src[it] = tgt_of[it] >= 0 ? src_slots[static_cast<int64_t>(tgt_of[it]) * src_ld + slot] : -1;
+  };
+  // Operator fragments are loaded one k step ahead of the MFMAs that use them,
+  // and the first step of the next offset during the last step of this one,
+  // so the MFMAs of a step never wait on its own fragment loads.
+  constexpr int NK = NPART == 0 ? KP / 16 : KP / 32;  // k steps per offset
+  constexpr int NA = NPART == 0 ? 1 : NPART;         // fragments per row block
+  typedef typename std::conditional<NPART == 0, float4, bf16x8>::type AFrag;
+  auto a_load = [&](const void *__restrict__ T, int ks, AFrag (&dst)[2][NA]) {
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+      if constexpr (NPART == 0) {
+        // FP32: one 16-byte load per row block covers the step's four k
+        const float4 *__restrict__ A = static_cast<const float4 *>(T);
+        dst[rb][0] = A[(ks * (KP / 16) + 2 * wave + rb) * 64 + lane];
+      } else {
+        const uint16_t *__restrict__ A = static_cast<const uint16_t *>(T);
+#pragma unroll
+        for (int p = 0; p < NPART; ++p)
+          dst[rb][p] = *reinterpret_cast<const bf16x8 *>(
+              A + ((static_cast<int64_t>(p * (KP / 32) + ks) * (KP / 16) + 2 * wave + rb) * 64 +
+                   lane) * 8);
+      }
+    }
+  };
+  auto table_of = [&](int pc) -> const void * {
+    const int tix = smap[tl[pc]];
+    return NPART == 0
+               ? static_cast<const void *>(static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP)
+               : static_cast<const void *>(static_cast<const uint16_t *>(Tv) +
+                                           static_cast<int64_t>(tix) * 3 * KP * KP);
+  };
+  AFrag ac[2][NA];
   if (na > 0) {
-    m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[0], tid, src);
+    src_load(tl[0]);
     (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
-    if (na > 1) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[1], tid, src);
+    if (na > 1) src_load(tl[1]);
+    a_load(table_of(0), 0, ac);
   }
   for (int pc = 0; pc < na; ++pc) {
     __syncthreads();  // the previous offset's operands are consumed
@@ -585,73 +651,65 @@ __global__ __launch_bounds__(2 * KP) void m2l_kernel(const void *__restrict__ Ms
     // a plain barrier: the tile lists keep only offsets with a source, so no
     // block-wide "any source" vote (an LDS atomic per thread) is needed
     __syncthreads();
-    const int tix = smap[tl[pc]];
     if (pc + 1 < na) {
       (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
-      if (pc + 2 < na) m2l_src_load<KP, NPART>(s_tgt, src_slots, src_ld, tl[pc + 2], tid, src);
+      if (pc + 2 < na) src_load(tl[pc + 2]);
     }
-    {
+    const void *Tcur = table_of(pc);
+    const void *Tnext = pc + 1 < na ? table_of(pc + 1) : nullptr;
+#pragma unroll
+    for (int ks = 0; ks < NK; ++ks) {
+      AFrag an[2][NA];
+      if (ks + 1 < NK) a_load(Tcur, ks + 1, an);
+      else if (Tnext) a_load(Tnext, 0, an);
       if constexpr (NPART == 0) {
         const float *sF = reinterpret_cast<const float *>(s_raw);
-        const float4 *__restrict__ A =
-            reinterpret_cast<const float4 *>(static_cast<const float *>(Tv) + static_cast<int64_t>(tix) * KP * KP);
-#pragma unroll 1
-        for (int k16 = 0; k16 < KP / 16; ++k16) {
-          // steps j = 0..3 take k = 16 k16 + 4 kq + j (the tables' fragment order)
-          const float4 a0 = A[(k16 * (KP / 16) + 2 * wave) * 64 + lane];
-          const float4 a1 = A[(k16 * (KP / 16) + 2 * wave + 1) * 64 + lane];
-          float4 bv[4];
+        // steps j = 0..3 take k = 16 ks + 4 kq + j (the tables' fragment order)
+        float4 bv[4];
 #pragma unroll
-          for (int cb = 0; cb < 4; ++cb)
-            bv[cb] = *reinterpret_cast<const float4 *>(&sF[(r16 + 16 * cb) * LDF + 16 * k16 + 4 * kq]);
+        for (int cb = 0; cb < 4; ++cb)
+          bv[cb] = *reinterpret_cast<const float4 *>(&sF[(r16 + 16 * cb) * LDF + 16 * ks + 4 * kq]);
+        const float4 a0 = ac[0][0], a1 = ac[1][0];
 #pragma unroll
-          for (int cb = 0; cb < 4; ++cb) {
-            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, bv[cb].x, acc[0][cb], 0, 0, 0);
-            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, bv[cb].x, acc[1][cb], 0, 0, 0);
-            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, bv[cb].y, acc[0][cb], 0, 0, 0);
-            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, bv[cb].y, acc[1][cb], 0, 0, 0);
-            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, bv[cb].z, acc[0][cb], 0, 0, 0);
-            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, bv[cb].z, acc[1][cb], 0, 0, 0);
-            acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, bv[cb].w, acc[0][cb], 0, 0, 0);
-            acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, bv[cb].w, acc[1][cb], 0, 0, 0);
-          }
+        for (int cb = 0; cb < 4; ++cb) {
+          acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, bv[cb].x, acc[0][cb], 0, 0, 0);
+          acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, bv[cb].x, acc[1][cb], 0, 0, 0);
+          acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, bv[cb].y, acc[0][cb], 0, 0, 0);
+          acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, bv[cb].y, acc[1][cb], 0, 0, 0);
+          acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, bv[cb].z, acc[0][cb], 0, 0, 0);
+          acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, bv[cb].z, acc[1][cb], 0, 0, 0);
+          acc[0][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, bv[cb].w, acc[0][cb], 0, 0, 0);
+          acc[1][cb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, bv[cb].w, acc[1][cb], 0, 0, 0);
         }
       } else {
         const uint16_t *sB = reinterpret_cast<const uint16_t *>(s_raw);
-        const uint16_t *__restrict__ A =
-            static_cast<const uint16_t *>(Tv) + static_cast<int64_t>(tix) * 3 * KP * KP;
-#pragma unroll 1
-        for (int k0 = 0; k0 < KP; k0 += 32) {
-          const int kk = k0 + 8 * kq;
-          bf16x8 a[2][NPART];
+        const int kk = 32 * ks + 8 * kq;
 #pragma unroll
-          for (int rb = 0; rb < 2; ++rb)
+        for (int cb = 0; cb < 4; ++cb) {
+          bf16x8 b[NPART];
 #pragma unroll
-            for (int p = 0; p < NPART; ++p)
-              a[rb][p] = *reinterpret_cast<const bf16x8 *>(
-                  A + ((static_cast<int64_t>(p * (KP / 32) + k0 / 32) * (KP / 16) + 2 * wave + rb) * 64 +
-                       lane) * 8);
+          for (int p = 0; p < NPART; ++p)
+            b[p] = *reinterpret_cast<const bf16x8 *>(&sB[(p * kTileT + 16 * cb + r16) * LDB + kk]);
 #pragma unroll
-          for (int cb = 0; cb < 4; ++cb) {
-            bf16x8 b[NPART];
-#pragma unroll
-            for (int p = 0; p < NPART; ++p)
-              b[p] = *reinterpret_cast<const bf16x8 *>(&sB[(p * kTileT + 16 * cb + r16) * LDB + kk]);
-#pragma unroll
-            for (int rb = 0; rb < 2; ++rb) {
-              if constexpr (NPART == 3) {
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[2], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[1], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][2], b[0], acc[rb][cb], 0, 0, 0);
-              }
-              if constexpr (NPART >= 2) {
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[1], acc[rb][cb], 0, 0, 0);
-                acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][1], b[0], acc[rb][cb], 0, 0, 0);
-              }
-              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][0], b[0], acc[rb][cb], 0, 0, 0);
+          for (int rb = 0; rb < 2; ++rb) {
+            if constexpr (NPART == 3) {
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][0], b[2], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][1], b[1], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][2], b[0], acc[rb][cb], 0, 0, 0);
             }
+            if constexpr (NPART >= 2) {
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][0], b[1], acc[rb][cb], 0, 0, 0);
+              acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][1], b[0], acc[rb][cb], 0, 0, 0);
+            }
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ac[rb][0], b[0], acc[rb][cb], 0, 0, 0);
           }
         }
+      }
+      if (ks + 1 < NK || Tnext) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+          for (int p = 0; p < NA; ++p) ac[rb][p] = an[rb][p];
       }
     }
   }
@@ -712,6 +770,14 @@ __global__ __launch_bounds__(256) void tile_slots_kernel(const M2LTile *__restri
     __syncthreads();  // s_wave is rewritten by the next pass
   }
   if (threadIdx.x == 0) nlist[blockIdx.x] = base;
+}
+
+// box_sys[g] = system of box g of one level: the bits above the level's 3l
+// Morton bits of its key.
+__global__ void box_sys_kernel(int nbox, const uint64_t *__restrict__ keys, int bits,
+                               int32_t *__restrict__ box_sys) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbox) box_sys[i] = static_cast<int32_t>(keys[i] >> bits);
 }
 
 // child_tab[parent * 8 + octant] = global index of the child (one level).
@@ -905,8 +971,9 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const float *__restrict__ Lt, Frame fr, double s, float hc0,
     float hc1, float hc2, int accumulate, float *__restrict__ ox, float *__restrict__ oy,
-    float *__restrict__ oz) {
+    float *__restrict__ oz, SysMask mask) {
   const int b = blockIdx.x;
+  if (sys_skip(mask, b)) return;
   const int lane = threadIdx.x;
   double cx, cy, cz;
   box_centre(fr, keys[b], s, cx, cy, cz);
@@ -1138,11 +1205,13 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_field_ke
     const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
     int nslot, int accumulate, FieldConsts fc, const float4 *__restrict__ ftab, float hc0,
-    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz) {
+    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz,
+    SysMask mask) {
   constexpr int G = 64 / T;
   const int li = xcd_contiguous(nleaves);
   if (li < 0) return;
   const int b = leaves[li];
+  if (sys_skip(mask, b)) return;  // uniform over the workgroup, before any barrier
   const int lane = threadIdx.x & 63;
   // The full-leaf shape takes its near-pair coefficients from the table
   // (2 KB of LDS).  The sparse shapes keep the closed form: their waves wait
@@ -1617,7 +1686,7 @@ int excess2(int x, int y, int z) {
   return e(x) + e(y) + e(z);
 }
 
-// Boxes whose nearest points lie less than two leaf sides apart: every pair
+// Boxes whose nearest points lie at most two leaf sides apart: every pair
 // within 4h (leaf side 2h).  Ordered by that separation (excess2 is its square
 // in leaf units), so the boxes that may hold pairs within 4h form a prefix.
 std::vector<int3> near_offsets_hess() {
@@ -1706,6 +1775,7 @@ struct FmmState {
   DeviceBuffer box_first;    // int32 [nbox]: first particle (leaf) / first child (level-local, inner)
   DeviceBuffer box_count;    // int32 [nbox]: particles (leaf) / children (inner)
   DeviceBuffer box_parent;   // int32 [nbox] global parent index
+  DeviceBuffer box_sys;      // int32 [nbox] system of each box (batch plans)
   DeviceBuffer nearF, nearH; // leaf lists
   DeviceBuffer cell_grid;    // int32 [8^l]: cell -> box of one level (list building)
   DeviceBuffer rngF, rngH;   // int2 merged near ranges [nleaf][nslot]
@@ -2360,6 +2430,16 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     }
     MF_HIP(hipStreamSynchronize(ctx->stream));  // host tile vectors go out of scope
   }
+  if (nsys > 1) {
+    MF_TRY(need<int32_t>(F.box_sys, F.nbox));
+    for (int l = 0; l <= D; ++l)
+      if (cnt[l] > 0) {
+        hipLaunchKernelGGL(box_sys_kernel, dim3(grid_for(cnt[l], 256)), dim3(256), 0, ctx->stream,
+                           cnt[l], F.box_keys.as<uint64_t>() + F.base[l], 3 * l,
+                           F.box_sys.as<int32_t>() + F.base[l]);
+        MF_LAUNCH_CHECK();
+      }
+  }
   MF_TRY(need<float>(F.Mt, static_cast<int64_t>(F.nbox) * F.KP));
   MF_TRY(need<float>(F.Lt, static_cast<int64_t>(F.nbox) * F.KP));
   MF_TRY(need<uint16_t>(F.Mb, static_cast<int64_t>(F.nbox) * 3 * F.KP));
@@ -2470,13 +2550,13 @@ namespace {
 template <int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
                   int ntiles, const int32_t *src, int nslot, const int32_t *tlist,
-                  const int32_t *tn) {
+                  const int32_t *tn, SysMask mask) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
 #define MF_M2L_ARGS                                                                           \
   ctx->stream, msrc, F.Lt.as<float>(), tables, tiles, ntiles,                                 \
       F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
-      tn, tab.slot_ld, 0
+      tn, tab.slot_ld, 0, mask
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
@@ -2515,7 +2595,8 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
 // their near field there, on the auxiliary stream, so that it shares the GPU
 // with the M2L (MFMA) rather than with the P2M (which is LDS-bound).
 template <int P, int KP>
-Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &before_m2l) {
+Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &before_m2l,
+                      SysMask mask) {
   mf_ctx *ctx = plan->ctx;
   FmmState &F = *plan->fmm;
   const FmmTables &tab = *F.tab;
@@ -2527,7 +2608,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     hipLaunchKernelGGL((p2m_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(),
-                       Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, F.Mt.as<float>());
+                       Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, F.Mt.as<float>(), mask);
     MF_LAUNCH_CHECK();
   }
   // M2M and L2L run on the M2L kernel, in the M2L's precision: FP32 MFMA, or
@@ -2553,7 +2634,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     const dim3 grid(8 * ((nt + 7) / 8)), block(2 * KP);
 #define MF_OCTANT_ARGS                                                                          \
   ctx->stream, msrc, out, tables, tiles, nt, boxes, src, src_ld,                               \
-      tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate
+      tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2, true>), grid, block, 0, MF_OCTANT_ARGS);
@@ -2588,17 +2669,17 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     KernelTimer timer(ctx, hess ? "m2l_force" : "m2l");
     if (!hess) {
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
-                            tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>()));
+                            tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>(), mask));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
       // force lists srcH (indexed by leaf = global index), the others srcF.
       const int lt = F.leaf_tile_start;
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
-                            tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>()));
+                            tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>(), mask));
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
                             F.srcH.as<int32_t>(), tab.nslotH,
                             F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * tab.slot_ld,
-                            F.tnH.as<int32_t>() + lt));
+                            F.tnH.as<int32_t>() + lt, mask));
     }
   }
   {
@@ -2642,6 +2723,8 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   const float h0 = hconst ? hconst[0] : 0.f, h1 = hconst ? hconst[1] : 0.f,
               h2 = hconst ? hconst[2] : 0.f;
   const int use_far = D >= 2 ? 1 : 0;
+  SysMask mask{};
+  if (plan->active_sys && F.nsys > 1) mask = SysMask{F.box_sys.as<int32_t>(), plan->active_sys};
   {
     // The near field (VALU) runs on the auxiliary stream, concurrently with
     // the M2L (MFMA); it writes H_const + near, and the L2P adds the far
@@ -2664,7 +2747,7 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
                        F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
                        F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
                        F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc,                \
-                       F.ftab->buf.as<float4>(), h0, h1, h2, ox, oy, oz);                        \
+                       F.ftab->buf.as<float4>(), h0, h1, h2, ox, oy, oz, mask);                  \
     MF_LAUNCH_CHECK();                                                                           \
   }
       MF_P2P_FIELD(64, 0)
@@ -2678,7 +2761,7 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       timer.stop();
       return serial ? Status::Ok() : ov.record();
     };
-    MF_TRY((upward_and_m2l<P, KP>(plan, false, near)));
+    MF_TRY((upward_and_m2l<P, KP>(plan, false, near, mask)));
     MF_TRY(ov.join());
   }
   if (use_far) {
@@ -2687,7 +2770,7 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
                        F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
                        F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
                        F.Lt.as<float>(), Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, h0, h1, h2,
-                       1, ox, oy, oz);
+                       1, ox, oy, oz, mask);
     MF_LAUNCH_CHECK();
   }
   return Status::Ok();
@@ -2738,7 +2821,7 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       timer.stop();
       return serial ? Status::Ok() : ov.record();
     };
-    MF_TRY((upward_and_m2l<P, KP>(plan, true, near)));
+    MF_TRY((upward_and_m2l<P, KP>(plan, true, near, SysMask{})));
     MF_TRY(ov.join());
   }
   Text9 t{};

@@ -42,6 +42,10 @@ struct mf_plan {
   // Optional per-particle factor (caller order) applied to the moments of FMM
   // field products; set by the batch solve for the span of its products.
   const float *weights = nullptr;
+  // Optional per-system flags (device int32, nsys): field products of a batch
+  // plan skip the boxes of systems whose flag is 0.  Set by the batch solve
+  // for the span of its iterations (converged systems are frozen there).
+  const int32_t *active_sys = nullptr;
   // Segments of the per-system reductions (built on first batch solve):
   // chunk c covers particles [seg[2c], seg[2c + 1]); system s owns chunks
   // [seg_first[s], seg_first[s + 1]).

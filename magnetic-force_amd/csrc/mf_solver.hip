@@ -791,6 +791,7 @@ __global__ void bcg_xr_kernel(int64_t n, const int32_t *__restrict__ sys_of,
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float a = static_cast<float>(alpha[sys_of[i]]);
+  if (a == 0.f) return;  // frozen system: its product entries may be stale
   x.x[i] = fmaf(a, p.x[i], x.x[i]);
   x.y[i] = fmaf(a, p.y[i], x.y[i]);
   x.z[i] = fmaf(a, p.z[i], x.z[i]);
@@ -930,10 +931,17 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   // products with per-particle kappa for the span of the solve
   struct WeightGuard {
     mf_plan *p;
-    ~WeightGuard() { p->weights = nullptr; }
+    ~WeightGuard() {
+      p->weights = nullptr;
+      p->active_sys = nullptr;
+    }
   } guard{plan};
   plan->weights = plan->kappa_part.as<float>();
   MF_TRY(residual(plan, 1.0, b, x, w, true, opts.zero_start != 0));
+  // products of the iterations skip converged systems, whose entries of g
+  // then keep earlier values: start g finite (the fill is a kernel, so the
+  // captured iteration holds no memset)
+  MF_TRY(vec_fill<float>(ctx, 3 * n, 0.f, w.g));
   if (!opts.zero_start) st->matvecs += 1;
   const CV3 r = cv3(w.r, n), p = cv3(w.p, n), ap = cv3(w.ap, n), g = cv3(w.g, n);
   MF_TRY(seg_dot(r, r, S.rrn));
@@ -941,6 +949,7 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   MF_LAUNCH_CHECK();
   double mx = 0.0;
   MF_TRY(read_scalar(ctx, S.maxratio, &mx));
+  plan->active_sys = S.active;
   // one CG iteration, enqueued on ctx->stream without host reads
   auto body = [&]() -> Status {
     MF_TRY(plan_apply_G(plan, 1.f, w.p, w.p + n, w.p + 2 * n, nullptr, w.g, w.g + n, w.g + 2 * n));
@@ -965,8 +974,8 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   // iteration is captured and replayed graph_iters times per convergence
   // check.  The capture and the replays run on a non-null stream (the
   // context's own when it works on the null stream), ordered with the
-  // context's stream by events.  Converged systems are frozen, so the extra
-  // replays past convergence only cost time.
+  // context's stream by events.  Converged systems are frozen and their boxes
+  // are skipped by the products, so replays past convergence cost little.
   struct GraphState {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

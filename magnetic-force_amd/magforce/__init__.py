@@ -117,6 +117,8 @@ _SIGS = {
                                _vp, _vp, _vp, _vp], _i),
     "mf_hessian_allpairs": ([_vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f,
                              _vp], _i),
+    "mf_hessian_allpairs9": ([_vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f,
+                              _vp], _i),
     "mf_force_allpairs": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _f, _vp, _vp, _vp, _vp], _i),
     "mf_force_allpairs_f64": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp], _i),
     "mf_force_subset_f64": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i64, _vp, _vp, _vp,
@@ -315,6 +317,21 @@ def hessian_allpairs(ctx: Context, tpos, spos, smom, h: float):
     _check(lib().mf_hessian_allpairs(ctx.h, t.shape[1], _ptr(t, 0), _ptr(t, 1), _ptr(t, 2),
                                      s.shape[1], _ptr(s, 0), _ptr(s, 1), _ptr(s, 2), _ptr(m, 0),
                                      _ptr(m, 1), _ptr(m, 2), float(h), _ptr(out)), ctx.h)
+    return out
+
+
+def hessian_allpairs9(ctx: Context, tpos, spos, smom, h: float):
+    """The force tensor as nine (3, 3)-ordered rows T_ab, a-major (the text9 /
+    dHext layout), shape (9, n_t)."""
+    import torch
+    t = _soa(tpos)
+    s = _soa(spos)
+    m = _soa(smom, s.shape[1])
+    out = torch.empty((9, t.shape[1]), dtype=torch.float32, device=t.device)
+    ptrs = (ctypes.c_void_p * 9)(*[out[k].data_ptr() for k in range(9)])
+    _check(lib().mf_hessian_allpairs9(ctx.h, t.shape[1], _ptr(t, 0), _ptr(t, 1), _ptr(t, 2),
+                                      s.shape[1], _ptr(s, 0), _ptr(s, 1), _ptr(s, 2), _ptr(m, 0),
+                                      _ptr(m, 1), _ptr(m, 2), float(h), ptrs), ctx.h)
     return out
 
 

@@ -101,6 +101,23 @@ void mfo_force_allpairs_f32ref(int64_t n, const float *x, const float *y, const 
                                const float *mx, const float *my, const float *mz, float h,
                                float *fx, float *fy, float *fz, int nthreads);
 
+/* FP64 CPU restatement of the reference's FMM field product (mf_cpufmm.c):
+ * out_i = hconst + (G m)_i with an ncrit-split octree (NDTree.hpp:413-522),
+ * the breadth-first dual traversal (DualTraversal.hpp:79-125) under the MAC
+ * theta^2 r0^2 > (r1 + r2)^2 and r0 - r1 - r2 > gap (FMMOptions.hpp:56-78),
+ * P = 6 expansion terms (DipoleBSpherical.hpp:95-230,
+ * SphericalMultipole3D.hpp:46-336) and the smoothed pair kernel over near
+ * leaf pairs.  The reference configures ncrit 384, theta 0.5, gap 3h
+ * (magnetic_force.cu:646-648).  Returns 0.  Its accuracy is pinned against
+ * mfo_field_allpairs in the CPU tests. */
+typedef struct mfo_fmm_stats {
+  int64_t boxes, leaves, levels;
+  int64_t near_box_pairs, far_box_pairs, near_particle_pairs;
+} mfo_fmm_stats;
+int mfo_fmm_field(int64_t n, const double *pos, const double *mom, double h, int ncrit,
+                  double theta, double gap, const double *hconst, double *out, int nthreads,
+                  mfo_fmm_stats *stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,9 @@ def lib():
         L.mfo_ref_nlcg_solve.restype = cint
         L.mfo_field_allpairs_f32ref.argtypes = [i64, fp, fp, fp, fp, fp, fp, flt, fp, fp, fp, cint]
         L.mfo_force_allpairs_f32ref.argtypes = [i64, fp, fp, fp, fp, fp, fp, flt, fp, fp, fp, cint]
+        L.mfo_fmm_field.argtypes = [i64, dp, dp, dbl, cint, dbl, dbl, dp, dp, cint,
+                                    ctypes.POINTER(FmmStats)]
+        L.mfo_fmm_field.restype = cint
         _lib = L
     return _lib
 
@@ -98,6 +101,29 @@ def hessian_pair(d, m, h, form="frame"):
     fn = lib().mfo_hessian_pair_frame if form == "frame" else lib().mfo_hessian_pair_iso
     fn(dp_, mp, h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     return out.reshape(3, 3)
+
+
+class FmmStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("boxes", "leaves", "levels", "near_box_pairs",
+                                               "far_box_pairs", "near_particle_pairs")]
+
+
+def fmm_field(pos, mom, h, ncrit=384, theta=0.5, gap=None, hconst=None, nthreads=0):
+    """The reference's FMM field product restated on the CPU in FP64
+    (mf_cpufmm.c): ncrit-split octree, dual traversal, P = 6.  gap defaults to
+    the reference's 3h.  Returns (field (n, 3), stats dict)."""
+    pos, pp = _d(pos)
+    mom, mp = _d(mom)
+    out = np.zeros((pos.shape[0], 3))
+    hc = None
+    if hconst is not None:
+        hcv, hc = _d(np.asarray(hconst, dtype=np.float64).reshape(3))
+    st = FmmStats()
+    lib().mfo_fmm_field(pos.shape[0], pp, mp, h, int(ncrit), float(theta),
+                        float(3.0 * h if gap is None else gap), hc,
+                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), nthreads,
+                        ctypes.byref(st))
+    return out, {k: getattr(st, k) for k, _ in FmmStats._fields_}
 
 
 def field_allpairs(tpos, spos, smom, h, hconst=None, nthreads=0):

@@ -116,6 +116,31 @@ def test_hessian_matches_oracle(ctx, oracle):
     assert rel_err(got.reshape(-1), want.reshape(-1)) < 1e-5
 
 
+@pytest.mark.parametrize("layout", ["near", "far_and_near", "large"])
+def test_hessian9_matches_oracle(ctx, oracle, layout):
+    """The tensor written straight into the nine dHext arrays (T_ab, a-major),
+    against the oracle: sources overlapping the targets (every tile mixed),
+    a second cluster beyond 4h (tiles culled to the dipole form), and enough
+    targets and sources to split the source tiles into chunks."""
+    import magforce
+    tpos = f32(jittered_block(6, 5, 4, seed=51))
+    spos = f32(jittered_block(4, 4, 3, seed=52) + [0.3, 0.2, 0.9])
+    if layout == "far_and_near":
+        spos = np.concatenate([spos, f32(jittered_block(5, 5, 5, seed=53) + [30.0, -12.0, 4.0])])
+    if layout == "large":
+        tpos = f32(jittered_block(24, 20, 12, seed=54))
+        spos = f32(jittered_block(20, 20, 16, seed=55) + [5.5, -3.0, 2.0])
+    m = f32(random_moments(spos.shape[0], seed=56))
+    want = oracle.hessian_allpairs(tpos, spos, m, 2.0)
+    got = magforce.hessian_allpairs9(ctx, dev(tpos), dev(spos), dev(m), 2.0).double().cpu().numpy()
+    got = got.T.reshape(-1, 3, 3)
+    assert rel_err(got.reshape(-1), want.reshape(-1)) < 1e-5
+    # symmetric, and the same values as the six-component entry point
+    assert np.array_equal(got[:, 0, 1], got[:, 1, 0]) and np.array_equal(got[:, 1, 2], got[:, 2, 1])
+    six = magforce.hessian_allpairs(ctx, dev(tpos), dev(spos), dev(m), 2.0).double().cpu().numpy()
+    assert np.array_equal(six[0], got[:, 0, 0]) and np.array_equal(six[5], got[:, 1, 2])
+
+
 def test_fp64_path_matches_oracle(ctx, oracle):
     import magforce
     import torch
