@@ -448,6 +448,9 @@ __global__ __launch_bounds__(64) void p2m_kernel(int nleaf, const int32_t *__res
 //    1 KB of consecutive memory (whole cache lines).  The multipoles are
 //    pre-split once per product (split_multipoles_kernel).
 constexpr int kTileT = 64;
+// Word 3 of a raw buffer resource on gfx950 (32-bit data format, no
+// swizzle): loads past the range return zeros.
+constexpr int kBufferRsrcWord3 = 0x00020000;
 constexpr int kRowM2M = 16, kRowL2L = 17, kSlotRows = 25;  // slot-table rows
 
 struct M2LTile {
@@ -484,24 +487,36 @@ struct M2LGather {
   static constexpr int IT = 64 * PER_T / (2 * KP);                    // chunks per thread
 };
 
+// The gather reads through a buffer resource whose range is the multipole
+// array: a missing source (-1) becomes an offset past the range, which the
+// hardware returns as zeros.  Every load is issued without a branch; a load
+// under a branch makes the compiler wait for all outstanding loads (these
+// and the operator prefetch) at the first use of any of them, which would
+// serialize the gather with the MFMAs.
 template <int KP, int NPART>
-__device__ __forceinline__ int m2l_mult_load(const void *__restrict__ Msrc,
-                                             const int (&src)[M2LGather<KP, NPART>::IT], int tid,
-                                             float4 (&v)[M2LGather<KP, NPART>::IT]) {
+__device__ __forceinline__ void m2l_mult_load(__amdgpu_buffer_rsrc_t msrc,
+                                              const int (&src)[M2LGather<KP, NPART>::IT], int tid,
+                                              float4 (&v)[M2LGather<KP, NPART>::IT]) {
   using G = M2LGather<KP, NPART>;
-  int any = 0;
+  constexpr uint32_t kBoxBytes = NPART == 0 ? KP * 4 : 3 * KP * 2;
+  typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #pragma unroll
   for (int it = 0; it < G::IT; ++it) {
-    const int r = (tid + it * 2 * KP) % G::PER_T;
-    v[it] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (src[it] >= 0) {
-      const float4 *base = reinterpret_cast<const float4 *>(Msrc) +
-                          static_cast<int64_t>(src[it]) * (NPART == 0 ? KP / 4 : 3 * KP / 8);
-      v[it] = base[r];  // BF16: chunk r of the box's [3][KP] parts block
-      any = 1;
-    }
+    const uint32_t r = static_cast<uint32_t>((tid + it * 2 * KP) % G::PER_T);
+    // BF16: chunk r of the box's [3][KP] parts block; src = -1 wraps to
+    // 2^32 - kBoxBytes, beyond any range the host accepts
+    const uint32_t off = static_cast<uint32_t>(src[it]) * kBoxBytes + 16u * r;
+    const u4 x = __builtin_amdgcn_raw_buffer_load_b128(msrc, static_cast<int>(off), 0, 0);
+    v[it] = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z),
+                        __uint_as_float(x.w));
   }
-  return any;
+}
+
+// Bytes of the multipole range the M2L gather may address: the wrapped offset
+// of a missing source (2^32 - box bytes) must lie beyond it.
+inline bool m2l_gather_range_ok(int64_t nbox, int KP, int npart) {
+  const int64_t box_bytes = npart == 0 ? KP * 4 : 3 * KP * 2;
+  return nbox * box_bytes <= (int64_t{1} << 32) - 2 * box_bytes;
 }
 
 // Pre-split multipoles: Mb[(box * 3 + p) * KP + d] = BF16 part p of Mt[box * KP + d].
@@ -533,7 +548,7 @@ constexpr int m2l_min_waves() {
 
 template <int KP, int NPART, bool OCTANT = false>
 __global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(m2l_min_waves<KP, NPART>())))
-void m2l_kernel(const void *__restrict__ Msrc,
+void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
                                                           float *__restrict__ Lt,
                                                           const void *__restrict__ Tv,
                                                           const M2LTile *__restrict__ tiles,
@@ -588,13 +603,21 @@ void m2l_kernel(const void *__restrict__ Msrc,
   // so the source-slot loads of a later offset wait on no LDS read
   int tgt_of[GA::IT];
 #pragma unroll
-  for (int it = 0; it < GA::IT; ++it) tgt_of[it] = s_tgt[(tid + it * NT) / GA::PER_T];
+  for (int it = 0; it < GA::IT; ++it) {
+    const int t = s_tgt[(tid + it * NT) / GA::PER_T];
+    tgt_of[it] = t >= 0 ? t : s_tgt[0];
+  }
+  const __amdgpu_buffer_rsrc_t msrc_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void *>(Msrc), 0, static_cast<int>(msrc_bytes), kBufferRsrcWord3);
   int src[GA::IT];
   float4 v[GA::IT];
+  // Unconditional loads, for the reason given at m2l_mult_load: an empty
+  // column of a partial tile gathers the sources of the tile's first target,
+  // and its product is never stored.
   auto src_load = [&](int slot) {
 #pragma unroll
     for (int it = 0; it < GA::IT; ++it)
-      src[it] = tgt_of[it] >= 0 ? src_slots[static_cast<int64_t>(tgt_of[it]) * src_ld + slot] : -1;
+      src[it] = src_slots[static_cast<int64_t>(tgt_of[it]) * src_ld + slot];
   };
   // Operator fragments are loaded one k step ahead of the MFMAs that use them,
   // and the first step of the next offset during the last step of this one,
@@ -626,11 +649,13 @@ void m2l_kernel(const void *__restrict__ Msrc,
                : static_cast<const void *>(static_cast<const uint16_t *>(Tv) +
                                            static_cast<int64_t>(tix) * 3 * KP * KP);
   };
+  // The loads for offsets past the tile's last one re-read that offset
+  // (valid indices, results unused): no load of the loop sits under a branch.
   AFrag ac[2][NA];
   if (na > 0) {
     src_load(tl[0]);
-    (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
-    if (na > 1) src_load(tl[1]);
+    m2l_mult_load<KP, NPART>(msrc_rsrc, src, tid, v);
+    src_load(tl[min(1, na - 1)]);
     a_load(table_of(0), 0, ac);
   }
   for (int pc = 0; pc < na; ++pc) {
@@ -651,17 +676,19 @@ void m2l_kernel(const void *__restrict__ Msrc,
     // a plain barrier: the tile lists keep only offsets with a source, so no
     // block-wide "any source" vote (an LDS atomic per thread) is needed
     __syncthreads();
-    if (pc + 1 < na) {
-      (void)m2l_mult_load<KP, NPART>(Msrc, src, tid, v);
-      if (pc + 2 < na) src_load(tl[pc + 2]);
-    }
+    m2l_mult_load<KP, NPART>(msrc_rsrc, src, tid, v);
+    src_load(tl[min(pc + 2, na - 1)]);
+    // scheduling fences keep the prefetches where they are written: the
+    // scheduler would otherwise sink them next to their uses
+    __builtin_amdgcn_sched_barrier(0);
     const void *Tcur = table_of(pc);
-    const void *Tnext = pc + 1 < na ? table_of(pc + 1) : nullptr;
+    const void *Tnext = table_of(min(pc + 1, na - 1));
 #pragma unroll
     for (int ks = 0; ks < NK; ++ks) {
       AFrag an[2][NA];
       if (ks + 1 < NK) a_load(Tcur, ks + 1, an);
-      else if (Tnext) a_load(Tnext, 0, an);
+      else a_load(Tnext, 0, an);
+      __builtin_amdgcn_sched_barrier(0);
       if constexpr (NPART == 0) {
         const float *sF = reinterpret_cast<const float *>(s_raw);
         // steps j = 0..3 take k = 16 ks + 4 kq + j (the tables' fragment order)
@@ -705,12 +732,10 @@ void m2l_kernel(const void *__restrict__ Msrc,
           }
         }
       }
-      if (ks + 1 < NK || Tnext) {
 #pragma unroll
-        for (int rb = 0; rb < 2; ++rb)
+      for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-          for (int p = 0; p < NA; ++p) ac[rb][p] = an[rb][p];
-      }
+        for (int p = 0; p < NA; ++p) ac[rb][p] = an[rb][p];
     }
   }
 #pragma unroll
@@ -1060,16 +1085,7 @@ __device__ __forceinline__ void field_range_mixed(const SrcRec *__restrict__ rec
   }
 }
 
-// Workgroup -> list position with each XCD (round-robin dispatch over 8)
-// walking a contiguous range: neighbouring leaves share most of their near
-// sources, which then hit in that XCD's L2.  Returns -1 for the padding.
-__device__ __forceinline__ int xcd_contiguous(int n) {
-  const int per_xcd = (n + 7) / 8;
-  const int pos = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
-  return pos < n ? pos : -1;
-}
-
-// Targets [tb, te) of a leaf of cnt particles handled by the T-shaped kernel.
+// Targets [tb, te) of a leaf of cnt particles taken by the full-chunk path.
 template <int T>
 __device__ __forceinline__ void p2p_target_span(int cnt, int &tb, int &te) {
   const int full = cnt & ~63, r = cnt & 63;
@@ -1082,37 +1098,13 @@ __device__ __forceinline__ void p2p_target_span(int cnt, int &tb, int &te) {
   }
 }
 
-// Sum over the 64 / T lane groups of a wave (lanes t, t + T, t + 2T, ...).
-template <int T>
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int off = T; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
+constexpr int kLeafLists = 7;  // leaf shapes: full chunks (64) and remainders (32 .. 1)
 
-// Near-source stream of a sparse leaf (the T < 64 shapes).  The leaf's near
-// ranges, concatenated, are walked in windows of kStreamW records by the
-// kStreamWaves waves of the workgroup, wave w taking windows w, w + W, ...:
-// lane l loads stream positions w0 + kStreamR l + r (issued while the
-// previous window is evaluated), the window goes through the wave's LDS
-// buffer, and lane group g takes records g, g + G, ... of it.  A
-// range-by-range walk instead waits once per G sources on a load, and one
-// wave per leaf left a few leaves with long streams setting the launch time.
-constexpr int kStreamR = 2;              // records per lane per window
-constexpr int kStreamW = 64 * kStreamR;  // window length
-constexpr int kStreamWaves = 4;          // waves per sparse leaf
-constexpr int kLeafLists = 7;            // leaf shapes T = 64, 32, 16, 8, 4, 2, 1
-
-struct NearStream {
-  int pre[kNearH + 1];  // stream offset of each range; pre[nrange] = total
-  int beg[kNearH];      // first record of each range
-  // per wave, the window: record halves {x, y, z, -}, {mx, my, mz, -}
-  float4 win[kStreamWaves][2][kStreamW];
-};
-
-// Fills pre/beg from the leaf's ranges (one wave).
-__device__ __forceinline__ void near_stream_scan(NearStream &S, const int2 *__restrict__ rg,
-                                                 int nrange, int lane) {
+// Stream offsets of a leaf's near ranges, concatenated: pre[r] = total
+// length of ranges 0 .. r-1 (pre[nrange] = the stream's length), beg[r] =
+// first record of range r.  One wave.
+__device__ __forceinline__ void range_prefix(int *pre, int *beg, const int2 *__restrict__ rg,
+                                             int nrange, int lane) {
   int carry = 0;
   for (int c = 0; c < nrange; c += 64) {
     const int r = c + lane;
@@ -1120,7 +1112,7 @@ __device__ __forceinline__ void near_stream_scan(NearStream &S, const int2 *__re
     if (r < nrange) {
       const int2 q = rg[r];
       len = q.y - q.x;
-      S.beg[r] = q.x;
+      beg[r] = q.x;
     }
     int incl = len;
 #pragma unroll
@@ -1128,226 +1120,80 @@ __device__ __forceinline__ void near_stream_scan(NearStream &S, const int2 *__re
       const int v = __shfl_up(incl, off, 64);
       if (lane >= off) incl += v;
     }
-    if (r < nrange) S.pre[r] = carry + incl - len;
+    if (r < nrange) pre[r] = carry + incl - len;
     carry += __shfl(incl, 63, 64);
   }
-  if (lane == 0) S.pre[nrange] = carry;
+  if (lane == 0) pre[nrange] = carry;
 }
 
-// The scan by wave 0, then a workgroup barrier; returns the total.
-__device__ __forceinline__ int near_stream_setup(NearStream &S, const int2 *__restrict__ rg,
-                                                 int nrange, int lane, int wave) {
-  if (wave == 0) near_stream_scan(S, rg, nrange, lane);
-  __syncthreads();
-  return S.pre[nrange];
-}
-
-// Loads the window starting at w0 into (a, b).  Position i < total is record
-// beg[r] + i - pre[r] of the last range r with pre[r] <= i (empty ranges share
-// their successor's offset and are skipped by "last").  Lane l takes the
-// kStreamR consecutive positions w0 + kStreamR l + r: one binary search finds
-// the range of the first, and the others step forward from it
-// (pre[nrange] = total ends the step).
-__device__ __forceinline__ void near_stream_fetch(const NearStream &S, const SrcRec *__restrict__ rec,
-                                                  int nrange, int total, int w0, int lane,
-                                                  float4 (&a)[kStreamR], float4 (&b)[kStreamR]) {
-  const int i0 = w0 + kStreamR * lane;
-  if (i0 >= total) return;
+// The range holding stream position i: the last r with pre[r] <= i (an
+// empty range shares its successor's offset, so "last" skips it).
+__device__ __forceinline__ int range_of(const int *pre, int nrange, int i) {
   int lo = 0, hi = nrange - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (S.pre[mid] <= i0) lo = mid; else hi = mid - 1;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid - 1;
   }
-#pragma unroll
-  for (int r = 0; r < kStreamR; ++r) {
-    const int i = i0 + r;
-    if (i < total) {
-      while (S.pre[lo + 1] <= i) ++lo;
-      const float4 *p = reinterpret_cast<const float4 *>(rec + S.beg[lo] + (i - S.pre[lo]));
-      a[r] = p[0];
-      b[r] = p[1];
-    }
-  }
+  return lo;
 }
 
-// Publishes the fetched window in the wave's LDS buffer (after the previous
-// one is consumed).  Every wave of the workgroup calls it the same number of
-// times.
-__device__ __forceinline__ void near_stream_publish(NearStream &S, int wave, int lane,
-                                                    const float4 (&a)[kStreamR],
-                                                    const float4 (&b)[kStreamR]) {
-  __syncthreads();  // the previous window is consumed
+// Sum over the 64 lanes of a wave.
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
-  for (int r = 0; r < kStreamR; ++r) {
-    S.win[wave][0][kStreamR * lane + r] = a[r];
-    S.win[wave][1][kStreamR * lane + r] = b[r];
-  }
-  __syncthreads();
+  for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
-// P2P, field: out[perm] (+)= sum over the leaf's near ranges.  accumulate = 0
-// writes H_const + near (trees without a far field).  One workgroup per
-// leaf, in one of two shapes:
-//  * T = 64: one wave; one target per lane, every lane walks all sources
-//    (scalar loads), over the leaf's full chunks of 64 targets -- and over
-//    the remainder too when it holds more than 32;
-//  * T = 32, 16, 8, 4, 2, 1, for a remainder of T/2 < r <= T targets (a
-//    sparse leaf, or the tail of a leaf of 65..79 particles): kStreamWaves waves share the
-//    leaf's source stream window by window, and each wave is split into
-//    G = 64 / T groups of T lanes; every group holds all remainder targets
-//    and takes every G-th source of a window, the groups' partial sums are
-//    added across lanes and the waves' in wave order.  A short remainder
-//    then keeps (most of) the waves busy instead of idling 64 - r lanes.
-// leaves: the leaf indices this launch covers (blockIdx.x -> leaf).
-template <int T>
-__global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_field_kernel(
-    const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
-    const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
-    int nslot, int accumulate, FieldConsts fc, const float4 *__restrict__ ftab, float hc0,
-    float hc1, float hc2, float *__restrict__ ox, float *__restrict__ oy, float *__restrict__ oz,
-    SysMask mask) {
-  constexpr int G = 64 / T;
-  const int li = xcd_contiguous(nleaves);
-  if (li < 0) return;
-  const int b = leaves[li];
-  if (sys_skip(mask, b)) return;  // uniform over the workgroup, before any barrier
-  const int lane = threadIdx.x & 63;
-  // The full-leaf shape takes its near-pair coefficients from the table
-  // (2 KB of LDS).  The sparse shapes keep the closed form: their waves wait
-  // on source windows, and the extra LDS would cost them occupancy.
-  __shared__ float4 s_ftab[T == 64 ? 2 * kFieldTabN : 1];
-  if constexpr (T == 64) {
-    for (int i = lane; i < 2 * kFieldTabN; i += 64) s_ftab[i] = ftab[i];
-    __syncthreads();
+// One lane's share of a leaf's near-source stream, in blocks: block i0
+// holds positions i0, i0 + 64, ..., i0 + 64 (kLaneBlock - 1) (consecutive
+// lanes read consecutive records), and the lane loads all of them before
+// evaluating any, so that kLaneBlock loads are in flight at once.  The loads
+// are issued without a branch (a load under a branch makes the compiler wait
+// for every outstanding load at its join): a position past the stream's end
+// re-reads the last record and becomes a zero moment 1e18 away, whose pair
+// terms are all exactly zero.
+constexpr int kLaneBlock = 4;
+struct LaneStream {
+  const float4 *__restrict__ rec4;
+  const int *pre;
+  const int *beg;
+  int total, nmix;
+  int lo, nxt, base;
+  float4 pa[kLaneBlock], pb[kLaneBlock];  // records of the block
+  bool mixed[kLaneBlock];  // their ranges may hold pairs inside the smoothing support
+  __device__ __forceinline__ void start(int lane, int nrange) {
+    lo = range_of(pre, nrange, min(lane, total - 1));
+    nxt = pre[lo + 1];
+    base = beg[lo] - pre[lo];
   }
-  const int tl = T == 64 ? lane : lane % T;
-  const int g = T == 64 ? 0 : lane / T;
-  const int s0 = start[b], cnt = count[b];
-  int tb, te;
-  p2p_target_span<T>(cnt, tb, te);
-  const int2 nrb = nr[b];
-  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
-  const double cf = 1.0 / (4.0 * kPi);
-  if constexpr (T < 64) {
-    // at most T targets: one pass over the near-source stream, its windows
-    // shared out over the workgroup's waves
-    __shared__ NearStream S;
-    __shared__ double s_red[kStreamWaves - 1][T][3];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int nrange = nrb.x + nrb.y;
-    const int total = near_stream_setup(S, rg, nrange, lane, wave);
-    if (tb >= te) return;  // uniform, after the setup barrier
-    const int t = tb + tl;
-    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
-    const float tx = me.x, ty = me.y, tz = me.z;
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-    float4 wa[kStreamR], wb[kStreamR];
+  __device__ __forceinline__ void fetch(int i0) {
 #pragma unroll
-    for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int nwin = (total + kStreamW - 1) / kStreamW;
-    const int rounds = (nwin + kStreamWaves - 1) / kStreamWaves;  // uniform
-    near_stream_fetch(S, rec, nrange, total, wave * kStreamW, lane, wa, wb);
-    for (int rd = 0; rd < rounds; ++rd) {
-      const int w0 = (rd * kStreamWaves + wave) * kStreamW;
-      near_stream_publish(S, wave, lane, wa, wb);
-      const int wn = w0 + kStreamWaves * kStreamW;
-      if (wn < total) near_stream_fetch(S, rec, nrange, total, wn, lane, wa, wb);
-      const int n = w0 < total ? min(kStreamW, total - w0) : 0;
-      const float4 *__restrict__ wx = S.win[wave][0];
-      const float4 *__restrict__ wm = S.win[wave][1];
-      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
-#pragma unroll 2
-      for (int k = g; k < n; k += G) {
-        const float4 pa = wx[k], pb = wm[k];
-        const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
-        const float mx = pb.x, my = pb.y, mz = pb.z;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float ir2 = ir * ir;
-        float beta = ir2 * ir;
-        float alpha = beta * ir2;
-        if (r2 < fc.near2) field_near_coeffs(fc, r2, ir, alpha, beta);
-        const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-        const float s1 = dm * alpha;
-        a1x = fmaf(dx, s1, a1x);
-        a1y = fmaf(dy, s1, a1y);
-        a1z = fmaf(dz, s1, a1z);
-        a2x = fmaf(mx, beta, a2x);
-        a2y = fmaf(my, beta, a2y);
-        a2z = fmaf(mz, beta, a2z);
+    for (int k = 0; k < kLaneBlock; ++k) {
+      const int i = i0 + 64 * k;
+      const int ic = min(i, total - 1);
+      while (nxt <= ic) {
+        ++lo;
+        nxt = pre[lo + 1];
+        base = beg[lo] - pre[lo];
       }
-      acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
-      acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
-      acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
-    }
-    acc0 = group_sum<T>(acc0);
-    acc1 = group_sum<T>(acc1);
-    acc2 = group_sum<T>(acc2);
-    // the waves' partial sums, added in wave order
-    if (wave > 0 && g == 0) {
-      s_red[wave - 1][tl][0] = acc0;
-      s_red[wave - 1][tl][1] = acc1;
-      s_red[wave - 1][tl][2] = acc2;
-    }
-    __syncthreads();
-    if (wave == 0 && g == 0 && t < te) {
-#pragma unroll
-      for (int w = 0; w < kStreamWaves - 1; ++w) {
-        acc0 += s_red[w][tl][0];
-        acc1 += s_red[w][tl][1];
-        acc2 += s_red[w][tl][2];
-      }
-      const int o = perm[s0 + t];
-      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
-      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
-      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
-      ox[o] = static_cast<float>(b0 + cf * acc0);
-      oy[o] = static_cast<float>(b1 + cf * acc1);
-      oz[o] = static_cast<float>(b2 + cf * acc2);
-    }
-    return;
-  }
-  for (int t0 = tb; t0 < te; t0 += T) {
-    const int t = t0 + tl;
-    const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
-    const float tx = me.x, ty = me.y, tz = me.z;
-    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
-    for (int r = 0; r < nrb.x + nrb.y; ++r) {
-      const int2 q = rg[r];
-      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
-      if (r < nrb.x)
-        field_range_mixed<G>(rec, q.x + g, q.y, tx, ty, tz, fc, s_ftab, a1x, a1y, a1z, a2x, a2y,
-                             a2z);
-      else
-        field_range_far<G>(rec, q.x + g, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
-      acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
-      acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
-      acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
-    }
-    if constexpr (T < 64) {
-      acc0 = group_sum<T>(acc0);
-      acc1 = group_sum<T>(acc1);
-      acc2 = group_sum<T>(acc2);
-    }
-    if (g == 0 && t < te) {
-      const int o = perm[s0 + t];
-      const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
-      const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
-      const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
-      ox[o] = static_cast<float>(b0 + cf * acc0);
-      oy[o] = static_cast<float>(b1 + cf * acc1);
-      oz[o] = static_cast<float>(b2 + cf * acc2);
+      const int64_t j = static_cast<int64_t>(base) + ic;
+      const float4 a = rec4[2 * j], m = rec4[2 * j + 1];
+      const bool live = i < total;
+      const float away = live ? 0.f : 1e18f, w = live ? 1.f : 0.f;
+      pa[k] = make_float4(a.x + away, a.y + away, a.z + away, 0.f);
+      pb[k] = make_float4(m.x * w, m.y * w, m.z * w, 0.f);
+      mixed[k] = live && lo < nmix;
     }
   }
-}
+};
 
-// Leaf shapes for the near-field kernels.  List 0 (T = 64) holds every leaf
-// with a full chunk of 64 or a remainder of more than 32; lists 1..6 hold the
-// leaves whose remainder (cnt mod 64) is 17..32, 9..16, 5..8, 3..4, 2 or 1
-// (T = 32, 16, 8, 4, 2, 1: a remainder fills at least half of each lane
-// group).  A leaf can be in list 0 and in one of the others: they cover
-// disjoint targets.  Two (list, leaf) entries per leaf (list kLeafLists =
+// Leaf shapes for the near-field kernels.  List 0 holds every leaf with a
+// full chunk of 64 or a remainder of more than 32 (the one-target-per-lane
+// kernels); lists 1..6 hold the leaves whose remainder (cnt mod 64) is
+// 17..32, 9..16, 5..8, 3..4, 2 or 1 (the lanes-over-sources kernels, each
+// list with the pass size that fits it).  A leaf can be in list 0 and in one
+// of the others: they cover disjoint targets.  Two (list, leaf) entries per leaf (list kLeafLists =
 // none); a stable sort by list then gives every list in leaf (Morton) order.
 __global__ void classify_leaves_kernel(int nleaf, const int32_t *__restrict__ count,
                                        uint64_t *__restrict__ key, int32_t *__restrict__ leaf,
@@ -1433,8 +1279,6 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
   }
 }
 
-// P2P, force: F[perm] (+)= (sum over the leaf's near ranges of B[m_j]) . m_i.
-// Leaf shapes (T) as in p2p_field_kernel.
 // Force near-field sums over one contiguous source range, every STRIDE-th
 // source from j0.  NEAR: the range may hold pairs within 4h (fitted curves
 // below q = 4); otherwise every pair takes the far form, with no test.
@@ -1472,122 +1316,210 @@ __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int 
   }
 }
 
-template <int T>
-__global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_kernel(
-    const int32_t *__restrict__ leaves, int nleaves, const int32_t *__restrict__ start,
-    const int32_t *__restrict__ count, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const int2 *__restrict__ rng, const int2 *__restrict__ nr,
-    int nslot, HessConsts hc, int accumulate, float *__restrict__ fx, float *__restrict__ fy,
-    float *__restrict__ fz) {
-  constexpr int G = 64 / T;
-  const int li = xcd_contiguous(nleaves);
-  if (li < 0) return;  // uniform over the workgroup, before the barrier
-  __shared__ float4 s_tab[12];
-  if (threadIdx.x < 12) {
-    const float *row = &hc.curve[threadIdx.x / 3][(threadIdx.x % 3) * 4];
-    s_tab[threadIdx.x] = make_float4(row[0], row[1], row[2], row[3]);
+// ---------------------------------------------------------------------------
+// P2P: the near field of every leaf, in one launch per product over two leaf
+// lists.
+//  * Full chunks (near_*_full): one wave per leaf, one target per lane,
+//    every lane walking all sources (scalar loads), over the leaf's chunks
+//    of 64 targets -- and over its remainder too when that holds more than
+//    32.
+//  * Remainders (near_*_remainder): the at most 32 targets left after a
+//    leaf's full chunks (a sparse leaf, or the tail of a leaf of 65..96
+//    particles).  One wave per leaf, lanes over sources: the leaf's near
+//    ranges, concatenated, form one stream; lane l takes positions l, l +
+//    64, ... (LaneStream) and evaluates each of its sources against R
+//    remainder targets per pass.  The targets are wave-uniform (scalar
+//    loads; R = 8 for the field and 4 for the force keep them in scalar
+//    registers without spilling), every lane has work whatever the
+//    remainder's size, and the lanes' partial sums are added across the
+//    wave in FP64.
+// Each XCD takes its share of the remainders before its share of the full
+// chunks (xcd_two_lists): a remainder wave is long (its whole stream, one
+// wave) but there are few, and started first they finish under the full
+// chunks' work instead of trailing a launch of their own.  Ranges of the
+// leaf's mixed slots (nr.x, first) may hold pairs inside the smoothing
+// support.
+
+// Workgroup -> position in one of two leaf lists (remainders first, then
+// full chunks), each XCD (round-robin dispatch over 8) walking a contiguous
+// range of each list: neighbouring leaves share most of their near sources,
+// which then hit in that XCD's L2.  Returns -1 for the padding.
+__device__ __forceinline__ int xcd_two_lists(int nrem, int nfull, bool &rem) {
+  const int prem = (nrem + 7) / 8, pfull = (nfull + 7) / 8;
+  const int x = static_cast<int>(blockIdx.x % 8), k = static_cast<int>(blockIdx.x / 8);
+  rem = k < prem;
+  const int pos = rem ? x * prem + k : x * pfull + (k - prem);
+  return pos < (rem ? nrem : nfull) ? pos : -1;
+}
+
+// Remainder targets [tb, te) of a leaf of cnt particles (empty when the
+// full-chunk path takes the remainder).
+__device__ __forceinline__ void remainder_span(int cnt, int &tb, int &te) {
+  tb = cnt & ~63;
+  te = (cnt & 63) <= 32 ? cnt : tb;
+}
+
+struct FieldOut {
+  int accumulate;
+  float hc0, hc1, hc2;
+  float *__restrict__ ox;
+  float *__restrict__ oy;
+  float *__restrict__ oz;
+  __device__ __forceinline__ void put(int o, double g0, double g1, double g2) const {
+    const double cf = 1.0 / (4.0 * kPi);
+    const double b0 = accumulate ? static_cast<double>(ox[o]) : hc0;
+    const double b1 = accumulate ? static_cast<double>(oy[o]) : hc1;
+    const double b2 = accumulate ? static_cast<double>(oz[o]) : hc2;
+    ox[o] = static_cast<float>(b0 + cf * g0);
+    oy[o] = static_cast<float>(b1 + cf * g1);
+    oz[o] = static_cast<float>(b2 + cf * g2);
   }
-  __syncthreads();
-  const float *tab = reinterpret_cast<const float *>(s_tab);
-  const int b = leaves[li];
-  const int lane = threadIdx.x & 63;
-  const int tl = T == 64 ? lane : lane % T;
-  const int g = T == 64 ? 0 : lane / T;
-  const int s0 = start[b], cnt = count[b];
+};
+
+struct ForceOut {
+  int accumulate;
+  float *__restrict__ fx;
+  float *__restrict__ fy;
+  float *__restrict__ fz;
+  __device__ __forceinline__ void put(int o, double f0, double f1, double f2) const {
+    const double sfac = 3.0 * kMu0 / (4.0 * kPi);
+    fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
+    fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
+    fz[o] = static_cast<float>((accumulate ? static_cast<double>(fz[o]) : 0.0) + sfac * f2);
+  }
+};
+
+__device__ __forceinline__ void near_field_full(const SrcRec *__restrict__ rec,
+                                                const int32_t *__restrict__ perm, int s0, int cnt,
+                                                const int2 *__restrict__ rg, int2 nrb,
+                                                const FieldConsts &fc, const float4 *s_ftab,
+                                                int lane, const FieldOut &out) {
   int tb, te;
-  p2p_target_span<T>(cnt, tb, te);
-  const int2 nrb = nr[b];
-  const int nrange = nrb.x + nrb.y;
-  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
-  const double sfac = 3.0 * kMu0 / (4.0 * kPi);
-  if constexpr (T < 64) {
-    // at most T targets: one pass over the near-source stream, its windows
-    // shared out over the workgroup's waves
-    __shared__ NearStream S;
-    __shared__ double s_red[kStreamWaves - 1][T][3];
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int total = near_stream_setup(S, rg, nrange, lane, wave);
-    if (tb >= te) return;  // uniform, after the setup barrier
-    const int t = tb + tl;
+  p2p_target_span<64>(cnt, tb, te);
+  for (int t0 = tb; t0 < te; t0 += 64) {
+    const int t = t0 + lane;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
-    const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
-    double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
-    float4 wa[kStreamR], wb[kStreamR];
-#pragma unroll
-    for (int r = 0; r < kStreamR; ++r) wa[r] = wb[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int nwin = (total + kStreamW - 1) / kStreamW;
-    const int rounds = (nwin + kStreamWaves - 1) / kStreamWaves;  // uniform
-    near_stream_fetch(S, rec, nrange, total, wave * kStreamW, lane, wa, wb);
-    for (int rd = 0; rd < rounds; ++rd) {
-      const int w0 = (rd * kStreamWaves + wave) * kStreamW;
-      near_stream_publish(S, wave, lane, wa, wb);
-      const int wn = w0 + kStreamWaves * kStreamW;
-      if (wn < total) near_stream_fetch(S, rec, nrange, total, wn, lane, wa, wb);
-      const int n = w0 < total ? min(kStreamW, total - w0) : 0;
-      const float4 *__restrict__ wx = S.win[wave][0];
-      const float4 *__restrict__ wm = S.win[wave][1];
-      float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
-#pragma unroll 2
-      for (int k = g; k < n; k += G) {
-        const float4 pa = wx[k], pb = wm[k];
-        const float dx = tx - pa.x, dy = ty - pa.y, dz = tz - pa.z;
-        const float mx = pb.x, my = pb.y, mz = pb.z;
-        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-        const float ir = __builtin_amdgcn_rsqf(r2);
-        const float ir2 = ir * ir;
-        float Pc = ir2 * ir2 * ir;
-        float Rc = -5.f * Pc * ir2;
-        if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
-        const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-        const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
-        const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
-        const float e = fmaf(Rc, dm * dn, Pc * mn);
-        const float f = Pc * dn;
-        dx0 = fmaf(dx, e, dx0);
-        dy0 = fmaf(dy, e, dy0);
-        dz0 = fmaf(dz, e, dz0);
-        mx0 = fmaf(mx, f, mx0);
-        my0 = fmaf(my, f, my0);
-        mz0 = fmaf(mz, f, mz0);
-        n0 = fmaf(Pc, dm, n0);
-      }
-      ad0 += dx0;
-      ad1 += dy0;
-      ad2 += dz0;
-      am0 += mx0;
-      am1 += my0;
-      am2 += mz0;
-      an += n0;
+    const float tx = me.x, ty = me.y, tz = me.z;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (int r = 0; r < nrb.x + nrb.y; ++r) {
+      const int2 q = rg[r];
+      float a1x = 0.f, a1y = 0.f, a1z = 0.f, a2x = 0.f, a2y = 0.f, a2z = 0.f;
+      if (r < nrb.x)
+        field_range_mixed<1>(rec, q.x, q.y, tx, ty, tz, fc, s_ftab, a1x, a1y, a1z, a2x, a2y, a2z);
+      else
+        field_range_far<1>(rec, q.x, q.y, tx, ty, tz, a1x, a1y, a1z, a2x, a2y, a2z);
+      acc0 += 3.0 * static_cast<double>(a1x) - static_cast<double>(a2x);
+      acc1 += 3.0 * static_cast<double>(a1y) - static_cast<double>(a2y);
+      acc2 += 3.0 * static_cast<double>(a1z) - static_cast<double>(a2z);
     }
-    double f0 = ad0 + am0 + static_cast<double>(nx) * an;
-    double f1 = ad1 + am1 + static_cast<double>(ny) * an;
-    double f2 = ad2 + am2 + static_cast<double>(nz) * an;
-    f0 = group_sum<T>(f0);
-    f1 = group_sum<T>(f1);
-    f2 = group_sum<T>(f2);
-    // the waves' partial sums, added in wave order
-    if (wave > 0 && g == 0) {
-      s_red[wave - 1][tl][0] = f0;
-      s_red[wave - 1][tl][1] = f1;
-      s_red[wave - 1][tl][2] = f2;
-    }
-    __syncthreads();
-    if (wave == 0 && g == 0 && t < te) {
-#pragma unroll
-      for (int w = 0; w < kStreamWaves - 1; ++w) {
-        f0 += s_red[w][tl][0];
-        f1 += s_red[w][tl][1];
-        f2 += s_red[w][tl][2];
-      }
-      const int o = perm[s0 + t];
-      fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
-      fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
-      fz[o] = static_cast<float>((accumulate ? static_cast<double>(fz[o]) : 0.0) + sfac * f2);
-    }
-    return;
+    if (t < te) out.put(perm[s0 + t], acc0, acc1, acc2);
   }
-  for (int t0 = tb; t0 < te; t0 += T) {
-    const int t = t0 + tl;
+}
+
+template <int R>
+__device__ __forceinline__ void near_field_remainder(const SrcRec *__restrict__ rec,
+                                                     const int32_t *__restrict__ perm, int s0,
+                                                     int cnt, const int *s_pre, const int *s_beg,
+                                                     int nrange, int nmix, const FieldConsts &fc,
+                                                     const float4 *s_ftab, int lane,
+                                                     const FieldOut &out) {
+  int tb, te;
+  remainder_span(cnt, tb, te);
+  for (int t0 = tb; t0 < te; t0 += R) {
+    const int nt = min(R, te - t0);  // uniform
+    float tx[R], ty[R], tz[R], gx[R], gy[R], gz[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const SrcRec me = rec[s0 + t0 + min(t, nt - 1)];
+      tx[t] = me.x;
+      ty[t] = me.y;
+      tz[t] = me.z;
+      gx[t] = gy[t] = gz[t] = 0.f;
+    }
+    LaneStream st{reinterpret_cast<const float4 *>(rec), s_pre, s_beg, s_pre[nrange], nmix};
+    st.start(lane, nrange);
+    for (int i0 = lane; i0 < st.total; i0 += 64 * kLaneBlock) {
+      st.fetch(i0);
+#pragma unroll
+      for (int k = 0; k < kLaneBlock; ++k) {
+        const float4 pa = st.pa[k], pb = st.pb[k];
+        const bool mixed = st.mixed[k];
+        const float nmx = -pb.x, nmy = -pb.y, nmz = -pb.z;
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          if (t < nt) {
+            const float dx = tx[t] - pa.x, dy = ty[t] - pa.y, dz = tz[t] - pa.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float ir = __builtin_amdgcn_rsqf(r2);
+            const float ir2 = ir * ir;
+            float beta = ir2 * ir;
+            float alpha = beta * ir2;
+            if (mixed && r2 < fc.near2)
+              field_near_coeffs_tab(s_ftab, fc.tab_scale, r2, ir, alpha, beta);
+            // g += 3 (d . m) alpha d - beta m
+            const float s1 = 3.f * fmaf(dx, pb.x, fmaf(dy, pb.y, dz * pb.z)) * alpha;
+            gx[t] = fmaf(dx, s1, fmaf(nmx, beta, gx[t]));
+            gy[t] = fmaf(dy, s1, fmaf(nmy, beta, gy[t]));
+            gz[t] = fmaf(dz, s1, fmaf(nmz, beta, gz[t]));
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (t < nt) {
+        const double sx = wave_sum(static_cast<double>(gx[t]));
+        const double sy = wave_sum(static_cast<double>(gy[t]));
+        const double sz = wave_sum(static_cast<double>(gz[t]));
+        if (lane == 0) out.put(perm[s0 + t0 + t], sx, sy, sz);
+      }
+    }
+  }
+}
+
+// P2P, field: out[perm] (+)= sum over each leaf's near ranges; accumulate = 0
+// writes H_const + near (trees without a far field).  full / remain: the two
+// leaf lists (see above).
+template <int T>
+__global__ __launch_bounds__(64) void p2p_field_kernel(
+    const int32_t *__restrict__ full, int nfull, const int32_t *__restrict__ remain, int nrem,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm,
+    const int2 *__restrict__ rng, const int2 *__restrict__ nr, int nslot, FieldConsts fc,
+    const float4 *__restrict__ ftab, FieldOut out, SysMask mask) {
+  static_assert(T == 64, "one wave per leaf");
+  bool rem;
+  const int li = xcd_two_lists(nrem, nfull, rem);
+  if (li < 0) return;
+  const int b = rem ? remain[li] : full[li];
+  if (sys_skip(mask, b)) return;  // uniform over the workgroup, before the barrier
+  const int lane = threadIdx.x;
+  // near-pair coefficients from the table (2 KB of LDS); the remainder's
+  // stream offsets
+  __shared__ float4 s_ftab[2 * kFieldTabN];
+  __shared__ int s_pre[kNearF + 1], s_beg[kNearF];
+  for (int i = lane; i < 2 * kFieldTabN; i += 64) s_ftab[i] = ftab[i];
+  const int2 nrb = nr[b];
+  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
+  if (rem) range_prefix(s_pre, s_beg, rg, nrb.x + nrb.y, lane);
+  __syncthreads();
+  if (rem)
+    near_field_remainder<8>(rec, perm, start[b], count[b], s_pre, s_beg, nrb.x + nrb.y, nrb.x, fc,
+                            s_ftab, lane, out);
+  else
+    near_field_full(rec, perm, start[b], count[b], rg, nrb, fc, s_ftab, lane, out);
+}
+
+__device__ __forceinline__ void near_force_full(const SrcRec *__restrict__ rec,
+                                                const int32_t *__restrict__ perm, int s0, int cnt,
+                                                const int2 *__restrict__ rg, int2 nrb,
+                                                const HessConsts &hc, const float *tab, int lane,
+                                                const ForceOut &out) {
+  int tb, te;
+  p2p_target_span<64>(cnt, tb, te);
+  const int nrange = nrb.x + nrb.y;
+  for (int t0 = tb; t0 < te; t0 += 64) {
+    const int t = t0 + lane;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
     double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
@@ -1595,11 +1527,11 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_ke
       const int2 q = rg[r];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
       if (r < nrb.x)
-        force_range<G, true>(rec, q.x + g, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0,
-                             mx0, my0, mz0, n0);
+        force_range<1, true>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0, mx0,
+                             my0, mz0, n0);
       else
-        force_range<G, false>(rec, q.x + g, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0,
-                              mx0, my0, mz0, n0);
+        force_range<1, false>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0, mx0,
+                              my0, mz0, n0);
       ad0 += dx0;
       ad1 += dy0;
       ad2 += dz0;
@@ -1608,21 +1540,114 @@ __global__ __launch_bounds__(T == 64 ? 64 : 64 * kStreamWaves) void p2p_force_ke
       am2 += mz0;
       an += n0;
     }
-    double f0 = ad0 + am0 + static_cast<double>(nx) * an;
-    double f1 = ad1 + am1 + static_cast<double>(ny) * an;
-    double f2 = ad2 + am2 + static_cast<double>(nz) * an;
-    if constexpr (T < 64) {
-      f0 = group_sum<T>(f0);
-      f1 = group_sum<T>(f1);
-      f2 = group_sum<T>(f2);
+    if (t < te)
+      out.put(perm[s0 + t], ad0 + am0 + static_cast<double>(nx) * an,
+              ad1 + am1 + static_cast<double>(ny) * an, ad2 + am2 + static_cast<double>(nz) * an);
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void near_force_remainder(const SrcRec *__restrict__ rec,
+                                                     const int32_t *__restrict__ perm, int s0,
+                                                     int cnt, const int *s_pre, const int *s_beg,
+                                                     int nrange, int nmix, const HessConsts &hc,
+                                                     const float *tab, int lane,
+                                                     const ForceOut &out) {
+  int tb, te;
+  remainder_span(cnt, tb, te);
+  for (int t0 = tb; t0 < te; t0 += R) {
+    const int nt = min(R, te - t0);  // uniform
+    float tx[R], ty[R], tz[R], nx[R], ny[R], nz[R];
+    float ax[R], ay[R], az[R], an[R];
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      const SrcRec me = rec[s0 + t0 + min(t, nt - 1)];
+      tx[t] = me.x;
+      ty[t] = me.y;
+      tz[t] = me.z;
+      nx[t] = me.mx;
+      ny[t] = me.my;
+      nz[t] = me.mz;
+      ax[t] = ay[t] = az[t] = an[t] = 0.f;
     }
-    if (g == 0 && t < te) {
-      const int o = perm[s0 + t];
-      fx[o] = static_cast<float>((accumulate ? static_cast<double>(fx[o]) : 0.0) + sfac * f0);
-      fy[o] = static_cast<float>((accumulate ? static_cast<double>(fy[o]) : 0.0) + sfac * f1);
-      fz[o] = static_cast<float>((accumulate ? static_cast<double>(fz[o]) : 0.0) + sfac * f2);
+    LaneStream st{reinterpret_cast<const float4 *>(rec), s_pre, s_beg, s_pre[nrange], nmix};
+    st.start(lane, nrange);
+    for (int i0 = lane; i0 < st.total; i0 += 64 * kLaneBlock) {
+      st.fetch(i0);
+#pragma unroll
+      for (int k = 0; k < kLaneBlock; ++k) {
+        const float4 pa = st.pa[k];
+        const float mx = st.pb[k].x, my = st.pb[k].y, mz = st.pb[k].z;
+        const bool mixed = st.mixed[k];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+          if (t < nt) {
+            const float dx = tx[t] - pa.x, dy = ty[t] - pa.y, dz = tz[t] - pa.z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            const float ir = __builtin_amdgcn_rsqf(r2);
+            const float ir2 = ir * ir;
+            float Pc = ir2 * ir2 * ir;
+            float Rc = -5.f * Pc * ir2;
+            if (mixed && !(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
+            const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
+            const float dn = fmaf(dx, nx[t], fmaf(dy, ny[t], dz * nz[t]));
+            const float mn = fmaf(mx, nx[t], fmaf(my, ny[t], mz * nz[t]));
+            const float e = fmaf(Rc, dm * dn, Pc * mn);
+            const float f = Pc * dn;
+            ax[t] = fmaf(dx, e, fmaf(mx, f, ax[t]));
+            ay[t] = fmaf(dy, e, fmaf(my, f, ay[t]));
+            az[t] = fmaf(dz, e, fmaf(mz, f, az[t]));
+            an[t] = fmaf(Pc, dm, an[t]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < R; ++t) {
+      if (t < nt) {
+        const double sx = wave_sum(static_cast<double>(ax[t]));
+        const double sy = wave_sum(static_cast<double>(ay[t]));
+        const double sz = wave_sum(static_cast<double>(az[t]));
+        const double sn = wave_sum(static_cast<double>(an[t]));
+        if (lane == 0)
+          out.put(perm[s0 + t0 + t], sx + static_cast<double>(nx[t]) * sn,
+                  sy + static_cast<double>(ny[t]) * sn, sz + static_cast<double>(nz[t]) * sn);
+      }
     }
   }
+}
+
+// P2P, force: F[perm] (+)= (sum over each leaf's near ranges of B[m_j]) . m_i;
+// the two leaf lists as in p2p_field_kernel.
+template <int T>
+__global__ __launch_bounds__(64) void p2p_force_kernel(
+    const int32_t *__restrict__ full, int nfull, const int32_t *__restrict__ remain, int nrem,
+    const int32_t *__restrict__ start, const int32_t *__restrict__ count,
+    const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm,
+    const int2 *__restrict__ rng, const int2 *__restrict__ nr, int nslot, HessConsts hc,
+    ForceOut out) {
+  static_assert(T == 64, "one wave per leaf");
+  bool rem;
+  const int li = xcd_two_lists(nrem, nfull, rem);
+  if (li < 0) return;  // uniform over the workgroup, before the barrier
+  const int b = rem ? remain[li] : full[li];
+  const int lane = threadIdx.x;
+  __shared__ float4 s_tab[12];
+  __shared__ int s_pre[kNearH + 1], s_beg[kNearH];
+  if (lane < 12) {
+    const float *row = &hc.curve[lane / 3][(lane % 3) * 4];
+    s_tab[lane] = make_float4(row[0], row[1], row[2], row[3]);
+  }
+  const float *tab = reinterpret_cast<const float *>(s_tab);
+  const int2 nrb = nr[b];
+  const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
+  if (rem) range_prefix(s_pre, s_beg, rg, nrb.x + nrb.y, lane);
+  __syncthreads();
+  if (rem)
+    near_force_remainder<4>(rec, perm, start[b], count[b], s_pre, s_beg, nrb.x + nrb.y, nrb.x, hc,
+                            tab, lane, out);
+  else
+    near_force_full(rec, perm, start[b], count[b], rg, nrb, hc, tab, lane, out);
 }
 
 // Moments into the sorted records: rec[i].m = scale * weight[perm[i]] * m[perm[i]]
@@ -1783,6 +1808,12 @@ struct FmmState {
   DeviceBuffer leaf_lists;   // int32: the four leaf-shape lists back to back, each in leaf order
   int leaf_list_n[kLeafLists] = {};
   int leaf_list_off[kLeafLists] = {};
+  // lists 1 .. kLeafLists - 1 (the remainders) lie back to back
+  int leaf_lists_remainder() const {
+    int n = 0;
+    for (int k = 1; k < kLeafLists; ++k) n += leaf_list_n[k];
+    return n;
+  }
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
@@ -2553,8 +2584,12 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
                   const int32_t *tn, SysMask mask) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
+  const int npart = prec == MF_M2L_FP32 ? 0 : 3;
+  if (!m2l_gather_range_ok(F.nbox, KP, npart))
+    return Status::Err(MF_ERR_UNSUPPORTED, "too many boxes for the M2L gather's 32-bit range");
+  const uint32_t msrc_bytes = static_cast<uint32_t>(static_cast<int64_t>(F.nbox) * (npart ? 6 * KP : 4 * KP));
 #define MF_M2L_ARGS                                                                           \
-  ctx->stream, msrc, F.Lt.as<float>(), tables, tiles, ntiles,                                 \
+  ctx->stream, msrc, msrc_bytes, F.Lt.as<float>(), tables, tiles, ntiles,                     \
       F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
       tn, tab.slot_ld, 0, mask
   switch (prec) {
@@ -2632,8 +2667,12 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
                          const int32_t *boxes, const int32_t *src, int src_ld,
                          const int32_t *tlist, const int32_t *tn, int accumulate) -> Status {
     const dim3 grid(8 * ((nt + 7) / 8)), block(2 * KP);
+    if (!m2l_gather_range_ok(F.nbox, KP, npart))
+      return Status::Err(MF_ERR_UNSUPPORTED, "too many boxes for the M2L gather's 32-bit range");
+    const uint32_t msrc_bytes =
+        static_cast<uint32_t>(static_cast<int64_t>(F.nbox) * (npart ? 6 * KP : 4 * KP));
 #define MF_OCTANT_ARGS                                                                          \
-  ctx->stream, msrc, out, tables, tiles, nt, boxes, src, src_ld,                               \
+  ctx->stream, msrc, msrc_bytes, out, tables, tiles, nt, boxes, src, src_ld,                   \
       tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
@@ -2739,25 +2778,18 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       }
       KernelTimer timer(ctx, "p2p_field", ns);
       const int32_t *lists = F.leaf_lists.as<int32_t>();
-#define MF_P2P_FIELD(T, k)                                                                       \
-  if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_field_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)),          \
-                       dim3(T == 64 ? 64 : 64 * kStreamWaves),                                  \
-                       0, ns, lists + F.leaf_list_off[k],                                      \
-                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
-                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
-                       F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, 0, plan->fc,                \
-                       F.ftab->buf.as<float4>(), h0, h1, h2, ox, oy, oz, mask);                  \
-    MF_LAUNCH_CHECK();                                                                           \
-  }
-      MF_P2P_FIELD(64, 0)
-      MF_P2P_FIELD(32, 1)
-      MF_P2P_FIELD(16, 2)
-      MF_P2P_FIELD(8, 3)
-      MF_P2P_FIELD(4, 4)
-      MF_P2P_FIELD(2, 5)
-      MF_P2P_FIELD(1, 6)
-#undef MF_P2P_FIELD
+      {
+        const int nfull = F.leaf_list_n[0], nrem = F.leaf_lists_remainder();
+        const int grid = 8 * ((nrem + 7) / 8 + (nfull + 7) / 8);
+        if (grid > 0) {
+          hipLaunchKernelGGL((p2p_field_kernel<64>), dim3(grid), dim3(64), 0, ns, lists, nfull,
+                             lists + F.leaf_list_off[1], nrem, F.box_first.as<int32_t>(),
+                             F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                             F.rngF.as<int2>(), F.nrF.as<int2>(), kNearF, plan->fc,
+                             F.ftab->buf.as<float4>(), FieldOut{0, h0, h1, h2, ox, oy, oz}, mask);
+          MF_LAUNCH_CHECK();
+        }
+      }
       timer.stop();
       return serial ? Status::Ok() : ov.record();
     };
@@ -2800,24 +2832,18 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       }
       KernelTimer timer(ctx, "p2p_force", ns);
       const int32_t *lists = F.leaf_lists.as<int32_t>();
-#define MF_P2P_FORCE(T, k)                                                                       \
-  if (F.leaf_list_n[k] > 0) {                                                                    \
-    hipLaunchKernelGGL((p2p_force_kernel<T>), dim3(8 * ((F.leaf_list_n[k] + 7) / 8)),          \
-                       dim3(T == 64 ? 64 : 64 * kStreamWaves),                                  \
-                       0, ns, lists + F.leaf_list_off[k],                                      \
-                       F.leaf_list_n[k], F.box_first.as<int32_t>(),                             \
-                       F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),     \
-                       F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc, 0, fx, fy, fz);  \
-    MF_LAUNCH_CHECK();                                                                           \
-  }
-      MF_P2P_FORCE(64, 0)
-      MF_P2P_FORCE(32, 1)
-      MF_P2P_FORCE(16, 2)
-      MF_P2P_FORCE(8, 3)
-      MF_P2P_FORCE(4, 4)
-      MF_P2P_FORCE(2, 5)
-      MF_P2P_FORCE(1, 6)
-#undef MF_P2P_FORCE
+      {
+        const int nfull = F.leaf_list_n[0], nrem = F.leaf_lists_remainder();
+        const int grid = 8 * ((nrem + 7) / 8 + (nfull + 7) / 8);
+        if (grid > 0) {
+          hipLaunchKernelGGL((p2p_force_kernel<64>), dim3(grid), dim3(64), 0, ns, lists, nfull,
+                             lists + F.leaf_list_off[1], nrem, F.box_first.as<int32_t>(),
+                             F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                             F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc,
+                             ForceOut{0, fx, fy, fz});
+          MF_LAUNCH_CHECK();
+        }
+      }
       timer.stop();
       return serial ? Status::Ok() : ov.record();
     };
