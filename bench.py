@@ -61,6 +61,14 @@ def parse():
     p.add_argument("--rtol", type=float, default=1e-6)
     p.add_argument("--m2l", default="bf16x3", choices=["fp32", "bf16x6", "bf16x3"],
                    help="FMM M2L precision (bf16x3: the mixed-precision FMM)")
+    p.add_argument("--relax", default="bf16x1", choices=["none", "bf16x1", "bf16x3"],
+                   help="M2L precision of the CG products made once the relative residual "
+                        "is at or below --relax-rtol (inexact Krylov); none = every product "
+                        "at --m2l")
+    p.add_argument("--relax-rtol", type=float, default=3e-3)
+    p.add_argument("--batch-graph-iters", type=int, default=0,
+                   help="batch leg: iterations per captured-graph replay between host "
+                        "convergence checks (0 = plain launches, the faster here)")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the 96-droplet batch leg")
     p.add_argument("--no-compare", action="store_true",
@@ -84,13 +92,27 @@ def parse():
     return p.parse_args()
 
 
+def relax_kwargs(args):
+    """Solve options of the relaxed products (none: every product at --m2l)."""
+    import magforce
+    if args.relax == "none":
+        return {}
+    code = {"bf16x1": magforce.M2L_BF16X1, "bf16x3": magforce.M2L_BF16X3}[args.relax]
+    return {"relax_m2l_precision": code, "relax_rtol": args.relax_rtol}
+
+
+def relax_info(args):
+    return None if args.relax == "none" else {"m2l": args.relax, "below_rel_residual": args.relax_rtol}
+
+
 class Stepper:
     """Runs plan build + solve + force for one particle system."""
 
-    def __init__(self, ctx, system, mode, rtol, m2l=0, serial_near=False):
+    def __init__(self, ctx, system, mode, rtol, m2l=0, serial_near=False, relax=None):
         import torch
         import magforce
         self.ctx = ctx
+        self.relax = relax or {}  # relax_m2l_precision / relax_rtol of the solve
         self.s = system
         self.mode = mode
         self.rtol = rtol
@@ -110,7 +132,7 @@ class Stepper:
                                       m2l_precision=self.m2l, serial_near=self.serial_near)
         else:
             self.plan.update_positions(self.pos)
-        res = self.plan.solve(self.s.kappa, self.b, x0=None, rtol=self.rtol)
+        res = self.plan.solve(self.s.kappa, self.b, x0=None, rtol=self.rtol, **self.relax)
         self.x = res.x
         self.stats = res.stats
         self.plan.force(self.s.kappa * self.x, out=self.F)
@@ -333,7 +355,7 @@ def run_timestep(ctx, st: Stepper, steps: int, sigma_h: float = 0.02, seed: int 
     def step(pos):
         nonlocal x
         st.plan.update_positions(pos)
-        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol)
+        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol, **st.relax)
         x = res.x
         st.plan.force(kappa * x, out=st.F)
         return res.stats
@@ -347,7 +369,7 @@ def run_timestep(ctx, st: Stepper, steps: int, sigma_h: float = 0.02, seed: int 
         ev[2 * k + 1].record()
         st.plan.update_positions(poses[k + 1])
         ev[2 * k + 2].record()
-        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol)
+        res = st.plan.solve(kappa, st.b, x0=x, rtol=st.rtol, **st.relax)
         x = res.x
         st.plan.force(kappa * x, out=st.F)
         iters.append(res.stats)
@@ -355,7 +377,7 @@ def run_timestep(ctx, st: Stepper, steps: int, sigma_h: float = 0.02, seed: int 
     torch.cuda.synchronize()
     ms = ev[0].elapsed_time(ev[-1]) / steps
     rebuild = [ev[2 * k + 1].elapsed_time(ev[2 * k + 2]) for k in range(steps)]
-    cold = st.plan.solve(kappa, st.b, x0=None, rtol=st.rtol)
+    cold = st.plan.solve(kappa, st.b, x0=None, rtol=st.rtol, **st.relax)
     torch.cuda.synchronize()
     dx = float(torch.linalg.norm(x - cold.x) / torch.linalg.norm(cold.x))
     out = {"motion": f"N(0, ({sigma_h} h)^2) per axis per step", "steps": steps,
@@ -410,7 +432,8 @@ def run_droplets(ctx, args, m2l_code):
                                                m2l_precision=m2l_code, leaf_side=leaf_side)
         else:
             state["plan"].update_positions(pos)
-        res = state["plan"].solve(kappas, b, rtol=args.rtol, graph_iters=2)
+        res = state["plan"].solve(kappas, b, rtol=args.rtol, graph_iters=args.batch_graph_iters,
+                                  **relax_kwargs(args))
         state["x"], state["stats"] = res.x, res.stats
         state["plan"].force(res.x * kpart, out=state["F"])
 
@@ -429,9 +452,9 @@ def run_droplets(ctx, args, m2l_code):
            "n_min": min(sy.n for sy in systems), "n_max": max(sy.n for sy in systems),
            "ms_per_step": ms, "m2l_precision": args.m2l, "order": order,
            "leaf_side_h": leaf_side, "autotune": tuned,
-           "graph_iters": 2,
+           "graph_iters": args.batch_graph_iters, "relax": relax_info(args),
            "solver": {"iterations": st["iterations"], "matvecs": st["matvecs"],
-                      "converged": st["converged"],
+                      "relaxed_matvecs": st["relaxed_matvecs"], "converged": st["converged"],
                       "max_sys_rel_residual": max(st["sys_rel_residual"])},
            "plan": state["plan"].info()}
     if not args.no_check:
@@ -623,8 +646,8 @@ def main():
         wall = time.perf_counter() - t0
         ms = e0.elapsed_time(e1) / args.steps
         kt = {k: ctx.kernel_time(k) for k in
-              ("field", "force", "p2m", "m2m", "split", "m2l", "m2l_force", "l2l", "l2p_field",
-               "p2p_field", "l2p_force", "p2p_force")}
+              ("field", "force", "p2m", "m2m", "split", "m2l", "m2l_relaxed", "m2l_force", "l2l",
+               "l2p_field", "p2p_field", "l2p_force", "p2p_force")}
         ctx.enable_timing(False)
         return ms, wall, kt
 
@@ -632,7 +655,8 @@ def main():
     # field serialized on one stream (serial_near): each kernel then has the
     # GPU to itself.  --profile-only runs only that configuration, so the
     # rocprofv3 averages and the roofline describe the same launches.
-    kst = Stepper(ctx, system, mode, args.rtol, m2l_code, serial_near=True)
+    kst = Stepper(ctx, system, mode, args.rtol, m2l_code, serial_near=True,
+                  relax=relax_kwargs(args))
     ms_serial, _, ktimes = timed_steps(kst, True)
     log(f"kernel timing (near field serialized): {ms_serial:.2f} ms/step")
     if args.profile_only:
@@ -641,7 +665,7 @@ def main():
     kst.plan.close()
     del kst
     # The measured step: the near field overlaps the M2L (the default).
-    st = Stepper(ctx, system, mode, args.rtol, m2l_code)
+    st = Stepper(ctx, system, mode, args.rtol, m2l_code, relax=relax_kwargs(args))
     ms_step, wall, _ = timed_steps(st, False)
     log(f"timed: {args.steps} steps, {ms_step:.2f} ms/step (wall {wall / args.steps * 1e3:.2f})")
     matvecs = st.stats["matvecs"]
@@ -687,8 +711,8 @@ def main():
         roofline = cands[0]
         roofline["others"] = cands[1:]
         breakdown = {}
-        for k in ("p2m", "m2m", "split", "m2l", "m2l_force", "l2l", "l2p_field", "p2p_field",
-                  "l2p_force", "p2p_force"):
+        for k in ("p2m", "m2m", "split", "m2l", "m2l_relaxed", "m2l_force", "l2l", "l2p_field",
+                  "p2p_field", "l2p_force", "p2p_force"):
             ms, cnt = ktimes[k]
             breakdown[k] = {"ms_total": ms / args.steps, "launches": cnt // max(args.steps, 1)}
         roofline["per_step_breakdown"] = breakdown
@@ -765,9 +789,12 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{system.name} (n={n}, h={system.h}, chi={system.chi}, "
                                f"H=(0,0,1), CG rtol {args.rtol}, {mode_name}"
-                               + (f", M2L {args.m2l})" if mode == magforce.MODE_FMM else ")"),
+                               + (f", M2L {args.m2l}" + (f", products below residual "
+                                  f"{args.relax_rtol:g} at {args.relax})" if args.relax != "none"
+                                  else ")") if mode == magforce.MODE_FMM else ")"),
                    "mode": mode_name, "n": n, "matvecs_per_step": matvecs,
                    "m2l_precision": args.m2l if mode == magforce.MODE_FMM else None,
+                   "relax": relax_info(args) if mode == magforce.MODE_FMM else None,
                    "solver": st.stats},
         "error": check,
         "fp32_fmm": compare,

@@ -239,6 +239,14 @@ typedef struct mf_solve_opts {
                               of the inner correction solves; 0 = the plan's own */
   double inner_rtol; /* mf_plan_solve_f64: relative tolerance of each inner solve;
                         0 = 1e-5 (all-pairs plans), 1e-3 (FMM plans) */
+  int relax_m2l_precision; /* CG on an FMM plan (mf_plan_solve, mf_plan_solve_batch): 1 + the
+                              mf_m2l_precision of the products made once the relative residual
+                              (the largest over the systems of a batch) is at or below
+                              relax_rtol; 0 = off.  An error of the product made at residual
+                              r perturbs the final residual by about that error times r, so the
+                              late products may be less accurate (inexact Krylov). */
+  int reserved_solve;
+  double relax_rtol; /* see relax_m2l_precision; 0 = 1e-3 */
 } mf_solve_opts;
 
 typedef struct mf_solve_stats {
@@ -248,6 +256,8 @@ typedef struct mf_solve_stats {
   int refinements;
   double rel_residual;      /* ||b - A x|| / ||b|| from the solver recurrences */
   double rel_residual_true; /* recomputed with a fresh product at exit */
+  int relaxed_matvecs;      /* products made at relax_m2l_precision */
+  int reserved_stats;
 } mf_solve_stats;
 
 /* Solve (I - kappa G) x = b in place (x holds the start).  Replaces

@@ -546,7 +546,7 @@ constexpr int m2l_min_waves() {
   return (NPART == 3 || (NPART == 2 && KP != 128)) ? 1 : 3;
 }
 
-template <int KP, int NPART, bool OCTANT = false>
+template <int KP, int NPART, bool OCTANT = false, bool FENCE = false>
 __global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(m2l_min_waves<KP, NPART>())))
 void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
                                                           float *__restrict__ Lt,
@@ -556,6 +556,7 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
                                                           const int32_t *__restrict__ sorted_boxes,
                                                           const int32_t *__restrict__ src_slots,
                                                           int src_ld,
+                                                          int empty_idx,
                                                           const int32_t *__restrict__ slot_table,
                                                           int slot_ld,
                                                           const int32_t *__restrict__ tile_list,
@@ -603,21 +604,21 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
   // so the source-slot loads of a later offset wait on no LDS read
   int tgt_of[GA::IT];
 #pragma unroll
-  for (int it = 0; it < GA::IT; ++it) {
-    const int t = s_tgt[(tid + it * NT) / GA::PER_T];
-    tgt_of[it] = t >= 0 ? t : s_tgt[0];
-  }
+  for (int it = 0; it < GA::IT; ++it) tgt_of[it] = s_tgt[(tid + it * NT) / GA::PER_T];
   const __amdgpu_buffer_rsrc_t msrc_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<void *>(Msrc), 0, static_cast<int>(msrc_bytes), kBufferRsrcWord3);
   int src[GA::IT];
   float4 v[GA::IT];
   // Unconditional loads, for the reason given at m2l_mult_load: an empty
-  // column of a partial tile gathers the sources of the tile's first target,
-  // and its product is never stored.
-  auto src_load = [&](int slot) {
+  // column of a partial tile, and every column of an offset past the tile's
+  // last one, reads row empty_idx of the slot array, all -1 (no source: the
+  // gather returns zeros without touching memory).
+  auto src_load = [&](int slot, bool live) {
 #pragma unroll
-    for (int it = 0; it < GA::IT; ++it)
-      src[it] = src_slots[static_cast<int64_t>(tgt_of[it]) * src_ld + slot];
+    for (int it = 0; it < GA::IT; ++it) {
+      const int row = live && tgt_of[it] >= 0 ? tgt_of[it] : empty_idx;
+      src[it] = src_slots[static_cast<int64_t>(row) * src_ld + slot];
+    }
   };
   // Operator fragments are loaded one k step ahead of the MFMAs that use them,
   // and the first step of the next offset during the last step of this one,
@@ -649,13 +650,14 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
                : static_cast<const void *>(static_cast<const uint16_t *>(Tv) +
                                            static_cast<int64_t>(tix) * 3 * KP * KP);
   };
-  // The loads for offsets past the tile's last one re-read that offset
-  // (valid indices, results unused): no load of the loop sits under a branch.
+  // The loads for offsets past the tile's last one read that offset's
+  // operator and no sources (results unused): no load of the loop sits under
+  // a branch.
   AFrag ac[2][NA];
   if (na > 0) {
-    src_load(tl[0]);
+    src_load(tl[0], true);
     m2l_mult_load<KP, NPART>(msrc_rsrc, src, tid, v);
-    src_load(tl[min(1, na - 1)]);
+    src_load(tl[min(1, na - 1)], na > 1);
     a_load(table_of(0), 0, ac);
   }
   for (int pc = 0; pc < na; ++pc) {
@@ -677,10 +679,11 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
     // block-wide "any source" vote (an LDS atomic per thread) is needed
     __syncthreads();
     m2l_mult_load<KP, NPART>(msrc_rsrc, src, tid, v);
-    src_load(tl[min(pc + 2, na - 1)]);
-    // scheduling fences keep the prefetches where they are written: the
-    // scheduler would otherwise sink them next to their uses
-    __builtin_amdgcn_sched_barrier(0);
+    src_load(tl[min(pc + 2, na - 1)], pc + 2 < na);
+    // FENCE: scheduling fences keep the prefetches where they are written
+    // (the scheduler would otherwise sink them next to their uses).  Measured
+    // faster for the force M2L's tiles and slower for the field's.
+    if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
     const void *Tcur = table_of(pc);
     const void *Tnext = table_of(min(pc + 1, na - 1));
 #pragma unroll
@@ -688,7 +691,7 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
       AFrag an[2][NA];
       if (ks + 1 < NK) a_load(Tcur, ks + 1, an);
       else a_load(Tnext, 0, an);
-      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (FENCE) __builtin_amdgcn_sched_barrier(0);
       if constexpr (NPART == 0) {
         const float *sF = reinterpret_cast<const float *>(s_raw);
         // steps j = 0..3 take k = 16 ks + 4 kq + j (the tables' fragment order)
@@ -2294,8 +2297,18 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     for (int k = 1; k < kLeafLists; ++k) F.leaf_list_off[k] = F.leaf_list_off[k - 1] + F.leaf_list_n[k - 1];
   }
   // 5. M2L source slots (levels >= 2) and the class-sorted tile lists
-  MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox) * tab.nslotF));
-  MF_TRY(need<int32_t>(F.srcH, static_cast<int64_t>(nleaf) * tab.nslotH));
+  // one row past the last box of each slot array holds -1 throughout: the
+  // M2L kernel's "no source" row
+  MF_TRY(need<int32_t>(F.srcF, static_cast<int64_t>(F.nbox + 1) * tab.nslotF));
+  MF_TRY(need<int32_t>(F.srcH, static_cast<int64_t>(nleaf + 1) * tab.nslotH));
+  hipLaunchKernelGGL(grid_clear_kernel, dim3(grid_for(tab.nslotF, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<int64_t>(tab.nslotF),
+                     F.srcF.as<int32_t>() + static_cast<int64_t>(F.nbox) * tab.nslotF);
+  MF_LAUNCH_CHECK();
+  hipLaunchKernelGGL(grid_clear_kernel, dim3(grid_for(tab.nslotH, 256)), dim3(256), 0, ctx->stream,
+                     static_cast<int64_t>(tab.nslotH),
+                     F.srcH.as<int32_t>() + static_cast<int64_t>(nleaf) * tab.nslotH);
+  MF_LAUNCH_CHECK();
   for (int l = 2; l <= D; ++l) {
     if (cnt[l] == 0) continue;
     const int32_t *g = nullptr;
@@ -2389,10 +2402,15 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
                       F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * ld,
                       F.tnH.as<int32_t>() + lt));
     // M2M / L2L tiles
-    MF_TRY(need<int32_t>(F.child_tab, static_cast<int64_t>(F.nbox) * 8));
+    MF_TRY(need<int32_t>(F.child_tab, static_cast<int64_t>(F.nbox + 1) * 8));
     MF_TRY(need<int32_t>(F.box_ids, F.nbox));
-    MF_HIP(hipMemsetAsync(F.child_tab.get(), 0xff, sizeof(int32_t) * 8 * static_cast<size_t>(F.nbox),
-                          ctx->stream));
+    MF_HIP(hipMemsetAsync(F.child_tab.get(), 0xff,
+                          sizeof(int32_t) * 8 * static_cast<size_t>(F.nbox + 1), ctx->stream));
+    // box_parent[nbox] = -1: the L2L's "no source" row (the array has room:
+    // its capacity bounds the boxes of every level)
+    hipLaunchKernelGGL(grid_clear_kernel, dim3(1), dim3(64), 0, ctx->stream, int64_t{1},
+                       F.box_parent.as<int32_t>() + F.nbox);
+    MF_LAUNCH_CHECK();
     hipLaunchKernelGGL(iota_kernel, dim3(grid_for(F.nbox, 256)), dim3(256), 0, ctx->stream, F.nbox,
                        F.box_ids.as<int32_t>());
     MF_LAUNCH_CHECK();
@@ -2580,8 +2598,8 @@ namespace {
 
 template <int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
-                  int ntiles, const int32_t *src, int nslot, const int32_t *tlist,
-                  const int32_t *tn, SysMask mask) {
+                  int ntiles, const int32_t *src, int src_rows, int nslot, const int32_t *tlist,
+                  const int32_t *tn, SysMask mask, bool fence) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
   const int npart = prec == MF_M2L_FP32 ? 0 : 3;
@@ -2590,31 +2608,36 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
   const uint32_t msrc_bytes = static_cast<uint32_t>(static_cast<int64_t>(F.nbox) * (npart ? 6 * KP : 4 * KP));
 #define MF_M2L_ARGS                                                                           \
   ctx->stream, msrc, msrc_bytes, F.Lt.as<float>(), tables, tiles, ntiles,                     \
-      F.class_ids2.as<int32_t>(), src, nslot, tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist,  \
+      F.class_ids2.as<int32_t>(), src, nslot, src_rows, tab.slot_tab.as<int32_t>(),            \
+      tab.slot_ld, tlist,                                                                      \
       tn, tab.slot_ld, 0, mask
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
       const void *msrc = F.Mt.get();
-      hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_M2L_ARGS);
+      if (fence) hipLaunchKernelGGL((m2l_kernel<KP, 0, false, true>), grid, block, 0, MF_M2L_ARGS);
+      else hipLaunchKernelGGL((m2l_kernel<KP, 0>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     case MF_M2L_BF16X6: {
       const void *tables = tab.Tb.get();
       const void *msrc = F.Mb.get();
-      hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_M2L_ARGS);
+      if (fence) hipLaunchKernelGGL((m2l_kernel<KP, 3, false, true>), grid, block, 0, MF_M2L_ARGS);
+      else hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     case MF_M2L_BF16X3: {
       const void *tables = tab.Tb.get();
       const void *msrc = F.Mb.get();
-      hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
+      if (fence) hipLaunchKernelGGL((m2l_kernel<KP, 2, false, true>), grid, block, 0, MF_M2L_ARGS);
+      else hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     case MF_M2L_BF16X1: {
       const void *tables = tab.Tb.get();
       const void *msrc = F.Mb.get();
-      hipLaunchKernelGGL((m2l_kernel<KP, 1>), grid, block, 0, MF_M2L_ARGS);
+      if (fence) hipLaunchKernelGGL((m2l_kernel<KP, 1, false, true>), grid, block, 0, MF_M2L_ARGS);
+      else hipLaunchKernelGGL((m2l_kernel<KP, 1>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
     default:
@@ -2672,7 +2695,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     const uint32_t msrc_bytes =
         static_cast<uint32_t>(static_cast<int64_t>(F.nbox) * (npart ? 6 * KP : 4 * KP));
 #define MF_OCTANT_ARGS                                                                          \
-  ctx->stream, msrc, msrc_bytes, out, tables, tiles, nt, boxes, src, src_ld,                   \
+  ctx->stream, msrc, msrc_bytes, out, tables, tiles, nt, boxes, src, src_ld, F.nbox,          \
       tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
@@ -2705,20 +2728,25 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
   }
   MF_TRY(before_m2l());
   {
-    KernelTimer timer(ctx, hess ? "m2l_force" : "m2l");
+    // field M2L launches made at a precision other than the plan's own (the
+    // relaxed products of a solve) are timed apart
+    const bool own = plan->m2l_precision() == plan->opts.m2l_precision;
+    KernelTimer timer(ctx, hess ? "m2l_force" : own ? "m2l" : "m2l_relaxed");
     if (!hess) {
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
-                            tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>(), mask));
+                            F.nbox, tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>(), mask,
+                            false));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
       // force lists srcH (indexed by leaf = global index), the others srcF.
       const int lt = F.leaf_tile_start;
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
-                            tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>(), mask));
+                            F.nbox, tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>(), mask,
+                            true));
       MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
-                            F.srcH.as<int32_t>(), tab.nslotH,
+                            F.srcH.as<int32_t>(), F.count[D], tab.nslotH,
                             F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * tab.slot_ld,
-                            F.tnH.as<int32_t>() + lt, mask));
+                            F.tnH.as<int32_t>() + lt, mask, true));
     }
   }
   {

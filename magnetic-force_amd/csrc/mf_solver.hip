@@ -313,6 +313,27 @@ Status residual(mf_plan *plan, double kappa, CV3 b, V3 x, const Work &w, bool se
   return Status::Ok();
 }
 
+// The plan's M2L precision override, restored when the solve exits.
+struct M2LOverrideGuard {
+  mf_plan *p;
+  int saved;
+  explicit M2LOverrideGuard(mf_plan *plan) : p(plan), saved(plan->m2l_override) {}
+  ~M2LOverrideGuard() { p->m2l_override = saved; }
+};
+
+// Relaxed products of a CG solve (inexact Krylov): once the relative residual
+// is at or below relax_rtol, the remaining products of an FMM plan run at
+// relax_m2l_precision.  A product error e made at residual r perturbs the
+// final residual by about e r, so a coarse M2L late in the solve costs no
+// accuracy at the solve's tolerance; returns whether the switch happened.
+bool relax_products(mf_plan *plan, const mf_solve_opts &o, double rel_residual) {
+  if (o.relax_m2l_precision <= 0 || plan->opts.mode != MF_MODE_FMM) return false;
+  const double tau = o.relax_rtol > 0.0 ? o.relax_rtol : 1e-3;
+  if (rel_residual > tau) return false;
+  plan->m2l_override = o.relax_m2l_precision - 1;
+  return true;
+}
+
 Status solve_cg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_opts &o,
                 mf_solve_stats *st) {
   mf_ctx *ctx = plan->ctx;
@@ -329,12 +350,16 @@ Status solve_cg(mf_plan *plan, double kappa, CV3 b, V3 x, const mf_solve_opts &o
   MF_TRY(read_scalar(ctx, w.sc + S_RR, &rr));
   const double bn = std::sqrt(bb);
   const double tol = o.rtol * (bn > 0.0 ? bn : 1.0);
+  M2LOverrideGuard guard(plan);
+  bool relaxed = false;
   int it = 0;
   while (std::sqrt(rr) > tol && st->matvecs < o.maxit) {
+    if (!relaxed) relaxed = relax_products(plan, o, std::sqrt(rr) / (bn > 0.0 ? bn : 1.0));
     float *g = w.g;
     MF_TRY(plan_apply_G(plan, static_cast<float>(kappa), w.p, w.p + n, w.p + 2 * n, nullptr, g,
                         g + n, g + 2 * n));
     st->matvecs += 1;
+    if (relaxed) st->relaxed_matvecs += 1;
     hipLaunchKernelGGL(cg_ap_kernel, dim3(nb), dim3(kRedBlock), 0, ctx->stream, n, cv3(w.p, n),
                        cv3(g, n), v3(w.ap, n), w.partials);
     MF_LAUNCH_CHECK();
@@ -659,11 +684,7 @@ Status solve_f64(mf_plan *plan, double kappa, const double *bx, const double *by
   const double inner_rtol = opts.inner_rtol > 0.0 ? opts.inner_rtol : (fmm ? 1e-3 : 1e-5);
   // FMM plans: the residual product runs at the plan's own M2L precision, the
   // inner solves at inner_m2l_precision (the override is restored on every exit)
-  struct OverrideGuard {
-    mf_plan *p;
-    int saved;
-    ~OverrideGuard() { p->m2l_override = saved; }
-  } guard{plan, plan->m2l_override};
+  M2LOverrideGuard guard(plan);
   const int outer_prec = plan->m2l_override;
   const int inner_prec = opts.inner_m2l_precision > 0 ? opts.inner_m2l_precision - 1 : outer_prec;
   double bn = 0.0;
@@ -989,12 +1010,23 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   } gs;
   const int graph_iters = opts.graph_iters;
   hipStream_t work = ctx->stream ? ctx->stream : ctx->own_stream;
+  M2LOverrideGuard m2l_guard(plan);
+  bool relaxed = false;
   int it = 0;
   while (mx > rtol && st->matvecs < maxit) {
+    if (!relaxed && relax_products(plan, opts, mx)) {
+      relaxed = true;
+      // a captured iteration holds the old precision's kernels: capture anew
+      if (gs.exec) MF_HIP(hipGraphExecDestroy(gs.exec));
+      if (gs.graph) MF_HIP(hipGraphDestroy(gs.graph));
+      gs.exec = nullptr;
+      gs.graph = nullptr;
+    }
+    const int m0 = st->matvecs;
     if (graph_iters > 0 && it >= 1) {
       if (!gs.exec) {
-        MF_HIP(hipEventCreateWithFlags(&gs.ev[0], hipEventDisableTiming));
-        MF_HIP(hipEventCreateWithFlags(&gs.ev[1], hipEventDisableTiming));
+        if (!gs.ev[0]) MF_HIP(hipEventCreateWithFlags(&gs.ev[0], hipEventDisableTiming));
+        if (!gs.ev[1]) MF_HIP(hipEventCreateWithFlags(&gs.ev[1], hipEventDisableTiming));
         hipStream_t saved = ctx->stream;
         const bool timing = ctx->timing;
         MF_HIP(hipEventRecord(gs.ev[0], saved));
@@ -1024,6 +1056,7 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
       st->matvecs += 1;
       ++it;
     }
+    if (relaxed) st->relaxed_matvecs += st->matvecs - m0;
     MF_TRY(read_scalar(ctx, S.maxratio, &mx));
     if (!std::isfinite(mx)) return Status::Err(MF_ERR_NOT_CONVERGED, "CG residual is not finite");
   }

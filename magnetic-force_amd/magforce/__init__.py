@@ -77,13 +77,15 @@ class SolveOpts(ctypes.Structure):
                 ("rtol", ctypes.c_double), ("refine_fp64", ctypes.c_int),
                 ("true_residual", ctypes.c_int), ("zero_start", ctypes.c_int),
                 ("graph_iters", ctypes.c_int), ("inner_m2l_precision", ctypes.c_int),
-                ("inner_rtol", ctypes.c_double)]
+                ("inner_rtol", ctypes.c_double), ("relax_m2l_precision", ctypes.c_int),
+                ("reserved_solve", ctypes.c_int), ("relax_rtol", ctypes.c_double)]
 
 
 class SolveStats(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int), ("matvecs", ctypes.c_int),
                 ("converged", ctypes.c_int), ("refinements", ctypes.c_int),
-                ("rel_residual", ctypes.c_double), ("rel_residual_true", ctypes.c_double)]
+                ("rel_residual", ctypes.c_double), ("rel_residual_true", ctypes.c_double),
+                ("relaxed_matvecs", ctypes.c_int), ("reserved_stats", ctypes.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -464,14 +466,21 @@ class Plan:
         return out
 
     def solve(self, kappa: float, b, x0=None, method: int = SOLVE_CG, rtol: float = 1e-6,
-              maxit: int = 1000, restart: int = 30, true_residual: bool = False):
+              maxit: int = 1000, restart: int = 30, true_residual: bool = False,
+              relax_m2l_precision=None, relax_rtol: float = 0.0):
+        """relax_m2l_precision (CG on FMM plans): the M2L precision of the
+        products made once the relative residual is at or below relax_rtol
+        (0 = 1e-3); None = every product at the plan's own precision."""
         import torch
         bb = _soa(b, self.n)
         # x0 None: the solver starts from zero itself, saving one product
         x = torch.empty_like(bb) if x0 is None else _soa(x0, self.n).clone()
         opts = SolveOpts(method=method, restart=restart, maxit=maxit, rtol=rtol,
                          true_residual=1 if true_residual else 0,
-                         zero_start=1 if x0 is None else 0)
+                         zero_start=1 if x0 is None else 0,
+                         relax_m2l_precision=0 if relax_m2l_precision is None
+                         else int(relax_m2l_precision) + 1,
+                         relax_rtol=float(relax_rtol))
         st = SolveStats()
         _check(lib().mf_plan_solve(self.h_plan, float(kappa), _ptr(bb, 0), _ptr(bb, 1),
                                    _ptr(bb, 2), _ptr(x, 0), _ptr(x, 1), _ptr(x, 2),
@@ -526,14 +535,19 @@ class BatchPlan(Plan):
         self.pos = p
 
     def solve(self, kappas, b, x0=None, rtol: float = 1e-6, maxit: int = 1000,
-              graph_iters: int = 0):
+              graph_iters: int = 0, relax_m2l_precision=None, relax_rtol: float = 0.0):
+        """relax_m2l_precision / relax_rtol as in Plan.solve, the residual being
+        the largest relative residual over the systems."""
         import torch
         bb = _soa(b, self.n)
         if len(kappas) != self.nsys:
             raise ValueError("one kappa per system")
         x = torch.empty_like(bb) if x0 is None else _soa(x0, self.n).clone()
         opts = SolveOpts(method=SOLVE_CG, maxit=maxit, rtol=rtol,
-                         zero_start=1 if x0 is None else 0, graph_iters=graph_iters)
+                         zero_start=1 if x0 is None else 0, graph_iters=graph_iters,
+                         relax_m2l_precision=0 if relax_m2l_precision is None
+                         else int(relax_m2l_precision) + 1,
+                         relax_rtol=float(relax_rtol))
         st = SolveStats()
         kap = (ctypes.c_double * self.nsys)(*[float(k) for k in kappas])
         res = (ctypes.c_double * self.nsys)()

@@ -132,6 +132,48 @@ def test_fmm_refinement_vs_oracle(ctx, oracle, outer):
     assert rel_err(F, F_ref) < 1e-5
 
 
+def test_relaxed_cg_vs_oracle(ctx, oracle):
+    """CG with relaxed products: once the relative residual falls to 3e-3 the
+    remaining products run with one-part BF16 translations (a few 1e-4 of
+    the field).  An error e made at residual r moves the final residual by
+    about e r, so the solution stays on the BF16x3 solve's accuracy (within
+    the p = 10 truncation of the FP64 oracle), and the force with it.  A
+    relax threshold at the tolerance itself relaxes nothing, and a plan
+    without a far field (all-pairs) ignores the option."""
+    import magforce
+    from magforce import configs
+    s = configs.lattice8k()
+    pos = s.pos.astype(np.float64)
+    b = np.tile(s.hconst, (s.n, 1))
+    x_ref, _, _ = oracle.cg_solve(pos, s.h, s.kappa, b, rtol=1e-12)
+    F_ref = oracle.force_allpairs(pos, s.kappa * x_ref, s.h)
+    plan = magforce.Plan(ctx, dev(pos), s.h, mode=magforce.MODE_FMM,
+                         m2l_precision=magforce.M2L_BF16X3)
+    full = plan.solve(s.kappa, dev(b), rtol=1e-6)
+    g_before = host(plan.field(full.x))
+    res = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1,
+                     relax_rtol=3e-3, true_residual=True)
+    st = res.stats
+    print(f"relaxed cg: {st} (plain: {full.stats})")
+    assert st["converged"] == 1 and st["relaxed_matvecs"] >= 1
+    assert st["relaxed_matvecs"] < st["matvecs"]
+    assert st["rel_residual_true"] <= 2e-6
+    x = host(res.x)
+    assert rel_err(x, x_ref) < 5e-6
+    assert rel_err(x, host(full.x)) < 2e-6
+    F = host(plan.force(dev(s.kappa * x)))
+    assert rel_err(F, F_ref) < 1e-5
+    # the plan's own precision is back after the solve
+    assert rel_err(host(plan.field(full.x)), g_before) == 0.0
+    none = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1,
+                      relax_rtol=1e-7)
+    assert none.stats["relaxed_matvecs"] == 0
+    assert rel_err(host(none.x), host(full.x)) == 0.0
+    ap = magforce.Plan(ctx, dev(pos), s.h)
+    res_ap = ap.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1)
+    assert res_ap.stats["relaxed_matvecs"] == 0
+
+
 def test_bf16x1_product_accuracy(ctx, oracle):
     """The one-part BF16 translations alone: a few 1e-3 of the far field, so
     the product stays within 1e-2 of the oracle (a cheap, inexact operator for
