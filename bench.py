@@ -65,7 +65,7 @@ def parse():
                    help="M2L precision of the CG products made once the relative residual "
                         "is at or below --relax-rtol (inexact Krylov); none = every product "
                         "at --m2l")
-    p.add_argument("--relax-rtol", type=float, default=3e-3)
+    p.add_argument("--relax-rtol", type=float, default=1e-2)
     p.add_argument("--batch-graph-iters", type=int, default=0,
                    help="batch leg: iterations per captured-graph replay between host "
                         "convergence checks (0 = plain launches, the faster here)")

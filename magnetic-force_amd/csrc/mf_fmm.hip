@@ -1151,12 +1151,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One lane's share of a leaf's near-source stream, in blocks: block i0
 // holds positions i0, i0 + 64, ..., i0 + 64 (kLaneBlock - 1) (consecutive
 // lanes read consecutive records), and the lane loads all of them before
-// evaluating any, so that kLaneBlock loads are in flight at once.  The loads
+// evaluating any, so that kLaneBlock loads are in flight at once.  Two per
+// block and four targets per pass keep the fused kernels at 64 registers
+// or fewer (eight waves per SIMD for the full chunks), measured faster than
+// larger blocks or passes.  The loads
 // are issued without a branch (a load under a branch makes the compiler wait
 // for every outstanding load at its join): a position past the stream's end
 // re-reads the last record and becomes a zero moment 1e18 away, whose pair
 // terms are all exactly zero.
-constexpr int kLaneBlock = 4;
+constexpr int kLaneBlock = 2;
 struct LaneStream {
   const float4 *__restrict__ rec4;
   const int *pre;
@@ -1332,10 +1335,9 @@ __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int 
 //    ranges, concatenated, form one stream; lane l takes positions l, l +
 //    64, ... (LaneStream) and evaluates each of its sources against R
 //    remainder targets per pass.  The targets are wave-uniform (scalar
-//    loads; R = 8 for the field and 4 for the force keep them in scalar
-//    registers without spilling), every lane has work whatever the
-//    remainder's size, and the lanes' partial sums are added across the
-//    wave in FP64.
+//    loads; R = 4 keeps them in scalar registers), every lane has work
+//    whatever the remainder's size, and the lanes' partial sums are added
+//    across the wave in FP64.
 // Each XCD takes its share of the remainders before its share of the full
 // chunks (xcd_two_lists): a remainder wave is long (its whole stream, one
 // wave) but there are few, and started first they finish under the full
@@ -1507,7 +1509,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
   if (rem) range_prefix(s_pre, s_beg, rg, nrb.x + nrb.y, lane);
   __syncthreads();
   if (rem)
-    near_field_remainder<8>(rec, perm, start[b], count[b], s_pre, s_beg, nrb.x + nrb.y, nrb.x, fc,
+    near_field_remainder<4>(rec, perm, start[b], count[b], s_pre, s_beg, nrb.x + nrb.y, nrb.x, fc,
                             s_ftab, lane, out);
   else
     near_field_full(rec, perm, start[b], count[b], rg, nrb, fc, s_ftab, lane, out);

@@ -861,6 +861,13 @@ __global__ void bcg_p_kernel(int64_t n, const int32_t *__restrict__ sys_of,
   p.z[i] = fmaf(b, p.z[i], r.z[i]);
 }
 
+__global__ void kappa_of_system_kernel(int64_t n, const int32_t *__restrict__ sys_of,
+                                       const float *__restrict__ kappa_sys,
+                                       float *__restrict__ kappa_part) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) kappa_part[i] = kappa_sys[sys_of[i]];
+}
+
 Status build_segments(mf_plan *plan) {
   if (plan->nseg > 0) return Status::Ok();
   std::vector<int64_t> seg;
@@ -906,14 +913,23 @@ Status solve_batch(mf_plan *plan, const double *kappa, const float *bx, const fl
   const double rtol = opts.rtol > 0.0 ? opts.rtol : 1e-6;
   const int maxit = opts.maxit > 0 ? opts.maxit : 1000;
   MF_TRY(build_segments(plan));
-  // per-particle kappa (caller order), applied to the moments of every product
-  std::vector<float> kp(static_cast<size_t>(n));
-  for (int s = 0; s < nsys; ++s)
-    for (int64_t i = plan->sys_offsets[s]; i < plan->sys_offsets[s + 1]; ++i)
-      kp[static_cast<size_t>(i)] = static_cast<float>(kappa[s]);
-  MF_TRY(plan->kappa_part.reserve(sizeof(float) * static_cast<size_t>(n)));
-  MF_HIP(hipMemcpyAsync(plan->kappa_part.get(), kp.data(), sizeof(float) * kp.size(),
-                        hipMemcpyHostToDevice, ctx->stream));
+  // per-particle kappa (caller order), applied to the moments of every
+  // product: expanded on the device from the nsys values
+  {
+    std::vector<float> ks(static_cast<size_t>(nsys));
+    for (int s = 0; s < nsys; ++s) ks[static_cast<size_t>(s)] = static_cast<float>(kappa[s]);
+    void *pk = nullptr;
+    MF_TRY(workspace(ctx, "batch_kappa", sizeof(float) * ks.size(), &pk));
+    MF_HIP(hipMemcpyAsync(pk, ks.data(), sizeof(float) * ks.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
+    MF_TRY(plan->kappa_part.reserve(sizeof(float) * static_cast<size_t>(n)));
+    hipLaunchKernelGGL(kappa_of_system_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n,
+                       plan->sys_of.as<int32_t>(), static_cast<const float *>(pk),
+                       plan->kappa_part.as<float>());
+    MF_LAUNCH_CHECK();
+    // the host copy's source must outlive it
+    MF_HIP(hipStreamSynchronize(ctx->stream));
+  }
   Work w{};
   MF_TRY(get_work(ctx, n, w));
   void *pv = nullptr;
