@@ -280,12 +280,13 @@ __global__ void near_list_kernel(int nleaf, const uint64_t *__restrict__ leaf_ke
   const int x = c.x + o.x, y = c.y + o.y, z = c.z + o.z;
   int r = -1;
   if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells)
-    r = grid ? grid[morton(x, y, z)] : find_key(leaf_keys, nleaf, sys_bits | morton(x, y, z));
+    r = grid ? grid[sys_bits | morton(x, y, z)]
+             : find_key(leaf_keys, nleaf, sys_bits | morton(x, y, z));
   list[t] = r;
 }
 
-// Dense cell -> box grid of one level of a single-system plan: the level's
-// keys are plain Morton codes of 3l bits, so grid[key] = box (level-local),
+// Dense cell -> box grid of one level: the level's keys are the system
+// index above a Morton code of 3l bits, so grid[key] = box (level-local),
 // -1 for an empty cell.  The list kernels then look a neighbour up with one
 // load instead of a binary search over the level's keys.
 __global__ void grid_clear_kernel(int64_t n, int32_t *__restrict__ grid) {
@@ -316,7 +317,8 @@ __global__ void m2l_list_kernel(int nbox, int box_base, const uint64_t *__restri
   if (o.x != 0x7fffffff) {
     const int x = c.x - o.x, y = c.y - o.y, z = c.z - o.z;
     if (x >= 0 && y >= 0 && z >= 0 && x < cells && y < cells && z < cells) {
-      const int f = grid ? grid[morton(x, y, z)] : find_key(keys, nbox, sys_bits | morton(x, y, z));
+      const int f = grid ? grid[sys_bits | morton(x, y, z)]
+                         : find_key(keys, nbox, sys_bits | morton(x, y, z));
       r = f >= 0 ? box_base + f : -1;
     }
   }
@@ -2240,8 +2242,9 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
   // at most 2^24 cells per level; nullptr (binary search) otherwise.
   auto level_grid = [&](int l, const int32_t **out) -> Status {
     *out = nullptr;
-    if (nsys != 1 || 3 * l > 24 || cnt[l] == 0) return Status::Ok();
-    const int64_t ncell = 1LL << (3 * l);
+    // at most 2^24 cells (64 MB); wider levels take the binary search
+    if (3 * l + sys_bits > 24 || cnt[l] == 0) return Status::Ok();
+    const int64_t ncell = 1LL << (3 * l + sys_bits);
     MF_TRY(need<int32_t>(F.cell_grid, ncell));
     hipLaunchKernelGGL(grid_clear_kernel, dim3(grid_for(ncell, 256)), dim3(256), 0, ctx->stream,
                        ncell, F.cell_grid.as<int32_t>());
