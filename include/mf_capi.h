@@ -351,6 +351,16 @@ typedef enum mf_dtype {
 int mf_vec_convert(mf_ctx *ctx, int64_t n, int src_dtype, const void *src, int dst_dtype,
                    void *dst);
 
+/* ---- allocation fault injection ------------------------------------------ */
+/* No counterpart in the reference, which never checks its cudaMalloc results
+ * (cuda_abstract_vector.cuh:171-174).  The hook is called with the size of
+ * every device allocation the library is about to make; a nonzero return
+ * makes that allocation fail with MF_ERR_ALLOC, as a failed hipMalloc does.
+ * NULL removes it.  Process-wide.  For tests of the allocation-failure paths
+ * of the library and of its callers. */
+typedef int (*mf_alloc_hook)(uint64_t bytes, void *user);
+int mf_set_alloc_hook(mf_alloc_hook hook, void *user);
+
 /* ---- allocation-free variant ------------------------------------------------ */
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under hipGraph stream capture, tight serving loops, harnesses

@@ -1,9 +1,11 @@
 // mf_capi.cpp -- extern "C" entry points (include/mf_capi.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <new>
 #include <string>
@@ -107,6 +109,7 @@ Status plan_set_positions(mf_plan *plan, int64_t n, const float *x, const float 
   MF_ARG(n == 0 || (x && y && z), "positions must be non-null");
   MF_ARG(plan->nsys == 1 || n == plan->sys_offsets.back(),
          "a batch plan keeps its particle count (offsets[nsys])");
+  plan->ready = false;
   plan->n = n;
   plan->pos64_valid = false;
   const int64_t np = padded_sources(n);
@@ -116,6 +119,7 @@ Status plan_set_positions(mf_plan *plan, int64_t n, const float *x, const float 
   MF_TRY(pack_sources(ctx, n, x, y, z, nullptr, nullptr, nullptr, 0.f, plan->rec.as<SrcRec>()));
   MF_TRY(tile_boxes(ctx, n, plan->rec.as<SrcRec>(), plan->boxes.as<float4>()));
   if (plan->opts.mode == MF_MODE_FMM) MF_TRY(fmm_build(plan));
+  plan->ready = true;
   return Status::Ok();
 }
 
@@ -159,6 +163,36 @@ Status plan_force(mf_plan *plan, const float *mx, const float *my, const float *
                                fy, fz);
 }
 
+namespace {
+// mf_set_alloc_hook's hook and its argument, swapped together under a lock
+std::mutex g_hook_mu;
+mf_alloc_hook g_hook = nullptr;
+void *g_hook_user = nullptr;
+std::atomic<bool> g_hook_set{false};
+}  // namespace
+
+// Options of the solve entry points: tolerances finite and not negative,
+// precision codes in range.
+Status check_solve_opts(const mf_solve_opts *o) {
+  if (!o) return Status::Ok();
+  MF_ARG(std::isfinite(o->rtol) && o->rtol >= 0.0, "rtol must be finite and >= 0");
+  MF_ARG(std::isfinite(o->inner_rtol) && o->inner_rtol >= 0.0,
+         "inner_rtol must be finite and >= 0");
+  MF_ARG(std::isfinite(o->relax_rtol) && o->relax_rtol >= 0.0,
+         "relax_rtol must be finite and >= 0");
+  MF_ARG(o->inner_m2l_precision >= 0 && o->inner_m2l_precision <= MF_M2L_BF16X1 + 1,
+         "inner_m2l_precision out of range");
+  MF_ARG(o->relax_m2l_precision >= 0 && o->relax_m2l_precision <= MF_M2L_BF16X1 + 1,
+         "relax_m2l_precision out of range");
+  return Status::Ok();
+}
+
+bool alloc_refused(size_t bytes) {
+  if (!g_hook_set.load(std::memory_order_acquire)) return false;
+  std::lock_guard<std::mutex> lock(g_hook_mu);
+  return g_hook != nullptr && g_hook(static_cast<uint64_t>(bytes), g_hook_user) != 0;
+}
+
 }  // namespace mf
 
 using mf::Status;
@@ -166,6 +200,14 @@ using mf::Status;
 extern "C" {
 
 int mf_abi_version(void) { return MF_ABI_VERSION; }
+
+int mf_set_alloc_hook(mf_alloc_hook hook, void *user) {
+  std::lock_guard<std::mutex> lock(mf::g_hook_mu);
+  mf::g_hook = hook;
+  mf::g_hook_user = user;
+  mf::g_hook_set.store(hook != nullptr, std::memory_order_release);
+  return MF_OK;
+}
 
 const char *mf_status_string(int status) {
   switch (status) {
@@ -324,7 +366,7 @@ int mf_field_allpairs(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n_t >= 0 && n_s >= 0, "sizes must be >= 0");
-    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
     MF_ARG(n_t == 0 || (tx && ty && tz && ox && oy && oz), "target/output pointers are null");
     MF_ARG(n_s == 0 || (sx && sy && sz && mx && my && mz), "source pointers are null");
     MF_HIP(hipSetDevice(ctx->device));
@@ -358,7 +400,7 @@ int mf_field_allpairs_f64(mf_ctx *ctx, int64_t n_t, const double *tx, const doub
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n_t >= 0 && n_s >= 0, "sizes must be >= 0");
-    MF_ARG(h > 0.0, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.0, "h must be finite and > 0");
     MF_HIP(hipSetDevice(ctx->device));
     return mf::field_allpairs_f64(ctx, n_t, tx, ty, tz, n_s, sx, sy, sz, mx, my, mz, h, hconst,
                                   ox, oy, oz);
@@ -375,7 +417,7 @@ int hessian_entry(mf_ctx *ctx, int64_t n_t, const float *tx, const float *ty, co
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n_t >= 0 && n_s >= 0, "sizes must be >= 0");
-    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
     MF_ARG(n_t == 0 || (tx && ty && tz), "target pointers are null");
     for (int k = 0; k < 9; ++k) MF_ARG(n_t == 0 || sym[k] < 0 || dst[k], "output pointers are null");
     MF_ARG(n_s == 0 || (sx && sy && sz && mx && my && mz), "source pointers are null");
@@ -439,7 +481,7 @@ int mf_force_allpairs(mf_ctx *ctx, int64_t n, const float *x, const float *y, co
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0, "n must be >= 0");
-    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
     MF_ARG(n == 0 || (x && y && z && mx && my && mz && fx && fy && fz), "pointers are null");
     MF_HIP(hipSetDevice(ctx->device));
     const int64_t np = mf::padded_sources(n);
@@ -462,7 +504,7 @@ int mf_force_allpairs_f64(mf_ctx *ctx, int64_t n, const double *x, const double 
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0, "n must be >= 0");
-    MF_ARG(h > 0.0, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.0, "h must be finite and > 0");
     MF_HIP(hipSetDevice(ctx->device));
     return mf::force_allpairs_f64(ctx, n, x, y, z, mx, my, mz, h, fx, fy, fz);
   };
@@ -476,7 +518,7 @@ int mf_force_subset_f64(mf_ctx *ctx, int64_t n, const double *x, const double *y
   if (!ctx) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "ctx is null"));
   auto run = [&]() -> Status {
     MF_ARG(n >= 0 && n_t >= 0, "sizes must be >= 0");
-    MF_ARG(h > 0.0, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.0, "h must be finite and > 0");
     MF_ARG(n_t == 0 || (idx && fx && fy && fz), "pointers are null");
     MF_HIP(hipSetDevice(ctx->device));
     return mf::force_subset_f64(ctx, n, x, y, z, mx, my, mz, h, n_t, idx, fx, fy, fz);
@@ -489,7 +531,7 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
   if (!ctx || !out) return mf::finish(ctx, Status::Err(MF_ERR_INVALID_ARG, "ctx/out is null"));
   *out = nullptr;
   auto run = [&]() -> Status {
-    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
     MF_HIP(hipSetDevice(ctx->device));
     mf_plan *plan = new (std::nothrow) mf_plan();
     if (!plan) return Status::Err(MF_ERR_ALLOC, "out of host memory");
@@ -532,7 +574,7 @@ int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const fl
     for (int s = 0; s < nsys; ++s) MF_ARG(offsets[s + 1] >= offsets[s], "offsets must not decrease");
     MF_ARG(offsets[nsys] <= INT32_MAX, "batch too large");
     MF_ARG(opts && opts->mode == MF_MODE_FMM, "batch plans need mode MF_MODE_FMM");
-    MF_ARG(h > 0.f, "h must be > 0");
+    MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
     MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X1,
            "unknown M2L precision");
     MF_ARG(opts->serial_near == 0 || opts->serial_near == 1, "serial_near must be 0 or 1");
@@ -568,8 +610,11 @@ int mf_plan_solve_batch(mf_plan *plan, const double *kappa, const float *bx, con
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
   auto run = [&]() -> Status {
     MF_ARG(kappa, "kappa (one per system) is null");
+    MF_ARG(plan->ready, "the plan has no valid positions (its last update failed)");
+    for (int s = 0; s < plan->nsys; ++s) MF_ARG(std::isfinite(kappa[s]), "kappa must be finite");
     MF_ARG(plan->n == 0 || (bx && by && bz && xx && xy && xz), "pointers are null");
     MF_ARG(plan->opts.mode == MF_MODE_FMM, "batch solves need an FMM plan");
+    MF_TRY(mf::check_solve_opts(opts));
     MF_HIP(hipSetDevice(plan->ctx->device));
     mf_solve_opts o{};
     o.method = MF_SOLVE_CG;
@@ -606,6 +651,9 @@ int64_t mf_plan_size(const mf_plan *plan) { return plan ? plan->n : -1; }
 
 int mf_plan_get_info(const mf_plan *plan, mf_plan_info *info) {
   if (!plan || !info) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan/info is null"));
+  if (!plan->ready)
+    return mf::finish(plan->ctx, Status::Err(MF_ERR_INVALID_ARG,
+                                             "the plan has no valid positions (its last update failed)"));
   *info = mf_plan_info{};
   info->n = plan->n;
   info->mode = plan->opts.mode;
@@ -628,6 +676,7 @@ int mf_plan_field(mf_plan *plan, const float *mx, const float *my, const float *
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
   auto run = [&]() -> Status {
     MF_ARG(plan->n == 0 || (mx && my && mz && ox && oy && oz), "pointers are null");
+    MF_ARG(plan->ready, "the plan has no valid positions (its last update failed)");
     MF_HIP(hipSetDevice(plan->ctx->device));
     return mf::plan_apply_G(plan, 1.f, mx, my, mz, hconst, ox, oy, oz);
   };
@@ -639,6 +688,7 @@ int mf_plan_force(mf_plan *plan, const float *mx, const float *my, const float *
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
   auto run = [&]() -> Status {
     MF_ARG(plan->n == 0 || (mx && my && mz && fx && fy && fz), "pointers are null");
+    MF_ARG(plan->ready, "the plan has no valid positions (its last update failed)");
     MF_HIP(hipSetDevice(plan->ctx->device));
     return mf::plan_force(plan, mx, my, mz, text9, fx, fy, fz);
   };
@@ -651,6 +701,9 @@ int mf_plan_solve(mf_plan *plan, double kappa, const float *bx, const float *by,
   if (!plan) return mf::finish(nullptr, Status::Err(MF_ERR_INVALID_ARG, "plan is null"));
   auto run = [&]() -> Status {
     MF_ARG(plan->n == 0 || (bx && by && bz && xx && xy && xz), "pointers are null");
+    MF_ARG(std::isfinite(kappa), "kappa must be finite");
+    MF_ARG(plan->ready, "the plan has no valid positions (its last update failed)");
+    MF_TRY(mf::check_solve_opts(opts));
     MF_HIP(hipSetDevice(plan->ctx->device));
     mf_solve_opts o{};
     o.method = MF_SOLVE_CG;
@@ -672,10 +725,10 @@ int mf_plan_solve_f64(mf_plan *plan, double kappa, const double *b3n, double *x3
   auto run = [&]() -> Status {
     const int64_t n = plan->n;
     MF_ARG(n == 0 || (b3n && x3n), "pointers are null");
+    MF_ARG(plan->ready, "the plan has no valid positions (its last update failed)");
     MF_ARG(plan->nsys == 1, "mf_plan_solve_f64 takes single-system plans");
-    if (opts)
-      MF_ARG(opts->inner_m2l_precision >= 0 && opts->inner_m2l_precision <= MF_M2L_BF16X1 + 1,
-             "inner_m2l_precision out of range");
+    MF_ARG(std::isfinite(kappa), "kappa must be finite");
+    MF_TRY(mf::check_solve_opts(opts));
     MF_HIP(hipSetDevice(plan->ctx->device));
     mf_solve_opts o{};
     o.rtol = 1e-12;

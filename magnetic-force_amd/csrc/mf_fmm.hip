@@ -156,8 +156,10 @@ __global__ void bbox_partial_kernel(const int64_t *__restrict__ chunk, const Src
   __shared__ float sh[6][256];
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   const int64_t c0 = chunk[2 * blockIdx.x], c1 = chunk[2 * blockIdx.x + 1];
+  int bad = 0;  // a NaN coordinate (fminf/fmaxf would drop it)
   for (int64_t i = c0 + threadIdx.x; i < c1; i += 256) {
     const SrcRec r = rec[i];
+    bad |= (r.x != r.x) | (r.y != r.y) | (r.z != r.z);
     lo[0] = fminf(lo[0], r.x);
     lo[1] = fminf(lo[1], r.y);
     lo[2] = fminf(lo[2], r.z);
@@ -169,7 +171,7 @@ __global__ void bbox_partial_kernel(const int64_t *__restrict__ chunk, const Src
     sh[c][threadIdx.x] = lo[c];
     sh[3 + c][threadIdx.x] = hi[c];
   }
-  __syncthreads();
+  bad = __syncthreads_or(bad);  // also orders the writes above
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s)
       for (int c = 0; c < 3; ++c) {
@@ -178,7 +180,9 @@ __global__ void bbox_partial_kernel(const int64_t *__restrict__ chunk, const Src
       }
     __syncthreads();
   }
-  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+  // a NaN anywhere in the chunk: the partial box is NaN, which the host's
+  // finiteness check of the bounding box rejects
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = bad ? NAN : sh[threadIdx.x][0];
 }
 
 // key = system << 3D | Morton(leaf cell); sys_of == nullptr: one system.
@@ -2099,6 +2103,8 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
   std::vector<double> lo(3 * nsys, INFINITY), hi(3 * nsys, -INFINITY);
   for (int c = 0; c < nch; ++c)
     for (int a = 0; a < 3; ++a) {
+      if (!std::isfinite(part[c * 6 + a]) || !std::isfinite(part[c * 6 + 3 + a]))
+        return Status::Err(MF_ERR_INVALID_ARG, "particle positions must be finite");
       const int sy = ch_sys[c];
       lo[3 * sy + a] = std::min<double>(lo[3 * sy + a], part[c * 6 + a]);
       hi[3 * sy + a] = std::max<double>(hi[3 * sy + a], part[c * 6 + 3 + a]);

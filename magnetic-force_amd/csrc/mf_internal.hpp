@@ -52,6 +52,10 @@ struct Status {
   } while (0)
 
 // Owning device allocation.
+// The allocation hook of mf_set_alloc_hook (fault injection); consulted by
+// every DeviceBuffer growth.  Returns true when the hook refuses `bytes`.
+bool alloc_refused(size_t bytes);
+
 class DeviceBuffer {
  public:
   DeviceBuffer() = default;
@@ -78,6 +82,9 @@ class DeviceBuffer {
     if (bytes <= bytes_) return Status::Ok();
     release();
     size_t want = bytes < 256 ? 256 : bytes;
+    if (alloc_refused(want))
+      return Status::Err(MF_ERR_ALLOC, "allocation of " + std::to_string(want) +
+                                           " bytes refused by the allocation hook");
     hipError_t e = hipMalloc(&ptr_, want);
     if (e != hipSuccess) {
       ptr_ = nullptr;

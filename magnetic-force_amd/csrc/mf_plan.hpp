@@ -21,6 +21,10 @@ struct mf_plan {
   mf_plan_opts opts{};
   int64_t n = 0;
   double h = 0.0;
+  // positions set and (FMM) the tree built; false from the start of an update
+  // until it completes, so that a failed update leaves no half-built state
+  // for a product to read
+  bool ready = false;
   mf::FieldConsts fc{};
   mf::HessConsts hc{};
   // All-pairs state: packed records (positions + current moments) and tile boxes.

@@ -149,7 +149,12 @@ _SIGS = {
     "mf_field_allpairs_noalloc": ([_vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
                                    _f, _vp, _vp, _vp, _vp], _i),
     "mf_noalloc_scratch_bytes": ([ctypes.POINTER(_i64)], _i),
+    "mf_set_alloc_hook": ([_vp, _vp], _i),
 }
+
+# mf_alloc_hook: int (*)(uint64_t bytes, void *user)
+ALLOC_HOOK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint64, ctypes.c_void_p)
+_alloc_hook_ref = None
 
 
 def lib():
@@ -281,6 +286,19 @@ def field_allpairs_noalloc(ctx: Context, tpos, spos, smom, h: float, hconst=None
                                            _ptr(m, 0), _ptr(m, 1), _ptr(m, 2), float(h), hc,
                                            _ptr(out, 0), _ptr(out, 1), _ptr(out, 2)), ctx.h)
     return out
+
+
+def set_alloc_hook(fn=None):
+    """Installs fn(bytes) -> bool as the library's allocation hook (True
+    refuses that device allocation: MF_ERR_ALLOC); None removes it."""
+    global _alloc_hook_ref
+    if fn is None:
+        _check(lib().mf_set_alloc_hook(None, None))
+        _alloc_hook_ref = None
+        return
+    hook = ALLOC_HOOK(lambda nbytes, user: 1 if fn(int(nbytes)) else 0)
+    _check(lib().mf_set_alloc_hook(ctypes.cast(hook, ctypes.c_void_p), None))
+    _alloc_hook_ref = hook  # the C side keeps a pointer to it
 
 
 def noalloc_scratch_bytes() -> int:

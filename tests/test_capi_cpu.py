@@ -120,3 +120,13 @@ def test_droplets_workload():
 def test_example_binary_built():
     assert os.path.exists(magforce.EXAMPLE_PATH)
     assert os.access(magforce.EXAMPLE_PATH, os.X_OK)
+
+
+def test_alloc_hook_installs_and_clears_without_a_device():
+    """The fault-injection hook is process state: it can be set and removed
+    with no device (its effect is tested on the GPU)."""
+    import magforce
+    seen = []
+    magforce.set_alloc_hook(lambda nbytes: seen.append(nbytes) or False)
+    magforce.set_alloc_hook(None)
+    assert seen == []
