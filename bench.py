@@ -61,7 +61,7 @@ def parse():
     p.add_argument("--rtol", type=float, default=1e-6)
     p.add_argument("--m2l", default="bf16x3", choices=["fp32", "bf16x6", "bf16x3"],
                    help="FMM M2L precision (bf16x3: the mixed-precision FMM)")
-    p.add_argument("--relax", default="bf16x1", choices=["none", "bf16x1", "bf16x3"],
+    p.add_argument("--relax", default="bf16x1cut", choices=["none", "bf16x1", "bf16x1cut", "bf16x3"],
                    help="M2L precision of the CG products made once the relative residual "
                         "is at or below --relax-rtol (inexact Krylov); none = every product "
                         "at --m2l")
@@ -97,7 +97,8 @@ def relax_kwargs(args):
     import magforce
     if args.relax == "none":
         return {}
-    code = {"bf16x1": magforce.M2L_BF16X1, "bf16x3": magforce.M2L_BF16X3}[args.relax]
+    code = {"bf16x1": magforce.M2L_BF16X1, "bf16x1cut": magforce.M2L_BF16X1_CUT,
+            "bf16x3": magforce.M2L_BF16X3}[args.relax]
     return {"relax_m2l_precision": code, "relax_rtol": args.relax_rtol}
 
 

@@ -56,8 +56,14 @@ typedef enum mf_m2l_precision {
                         FP32 accumulation: FP32-level accuracy */
   MF_M2L_BF16X3 = 2, /* BF16 MFMA, 2 parts, 3 products: ~1e-5 relative per
                         translation (inner products of the mixed solve) */
-  MF_M2L_BF16X1 = 3  /* BF16 MFMA, 1 part, 1 product: ~4e-3 relative per
+  MF_M2L_BF16X1 = 3, /* BF16 MFMA, 1 part, 1 product: ~4e-3 relative per
                         translation (inner corrections of the refined solve) */
+  MF_M2L_BF16X1_CUT = 4 /* MF_M2L_BF16X1 with the translation cut to the
+                        16 x 32 blocks whose lowest local and multipole
+                        degrees add up to at most the expansion order (13 of 32
+                        blocks at order 10): ~1e-4 relative per product, for the
+                        late products of a relaxed solve (relax_m2l_precision).
+                        M2M and L2L run as MF_M2L_BF16X1. */
 } mf_m2l_precision;
 
 typedef enum mf_mode {

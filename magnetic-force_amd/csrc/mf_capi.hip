@@ -180,9 +180,9 @@ Status check_solve_opts(const mf_solve_opts *o) {
          "inner_rtol must be finite and >= 0");
   MF_ARG(std::isfinite(o->relax_rtol) && o->relax_rtol >= 0.0,
          "relax_rtol must be finite and >= 0");
-  MF_ARG(o->inner_m2l_precision >= 0 && o->inner_m2l_precision <= MF_M2L_BF16X1 + 1,
+  MF_ARG(o->inner_m2l_precision >= 0 && o->inner_m2l_precision <= MF_M2L_BF16X1_CUT + 1,
          "inner_m2l_precision out of range");
-  MF_ARG(o->relax_m2l_precision >= 0 && o->relax_m2l_precision <= MF_M2L_BF16X1 + 1,
+  MF_ARG(o->relax_m2l_precision >= 0 && o->relax_m2l_precision <= MF_M2L_BF16X1_CUT + 1,
          "relax_m2l_precision out of range");
   return Status::Ok();
 }
@@ -541,7 +541,7 @@ int mf_plan_create(mf_ctx *ctx, int64_t n, const float *x, const float *y, const
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown plan mode");
     }
-    if (plan->opts.m2l_precision < MF_M2L_FP32 || plan->opts.m2l_precision > MF_M2L_BF16X1) {
+    if (plan->opts.m2l_precision < MF_M2L_FP32 || plan->opts.m2l_precision > MF_M2L_BF16X1_CUT) {
       delete plan;
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
     }
@@ -575,7 +575,7 @@ int mf_plan_create_batch(mf_ctx *ctx, int nsys, const int64_t *offsets, const fl
     MF_ARG(offsets[nsys] <= INT32_MAX, "batch too large");
     MF_ARG(opts && opts->mode == MF_MODE_FMM, "batch plans need mode MF_MODE_FMM");
     MF_ARG(std::isfinite(h) && h > 0.f, "h must be finite and > 0");
-    MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X1,
+    MF_ARG(opts->m2l_precision >= MF_M2L_FP32 && opts->m2l_precision <= MF_M2L_BF16X1_CUT,
            "unknown M2L precision");
     MF_ARG(opts->serial_near == 0 || opts->serial_near == 1, "serial_near must be 0 or 1");
     MF_HIP(hipSetDevice(ctx->device));

@@ -762,6 +762,230 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Degree-cut one-part M2L (MF_M2L_BF16X1_CUT), for the late products of a
+// relaxed solve.  The translation L_j = sum_n T_jn M_n couples local degree j
+// with multipole degree n; its terms fall off like rho^(j + n) (rho = box
+// radius over separation, below 1/2 for the interaction lists here).  This
+// kernel keeps the 16 x 32 MFMA blocks (row block rb: local dofs 16 rb ..,
+// k step ks: multipole dofs 32 ks ..) whose lowest degrees add up to at most
+// P -- for P = 10, 13 of the 32 blocks -- and drops the rest, whose terms all
+// have j + n > P.  Dropped, they change a product by ~4e-5 relative on the
+// 1.15 M pool (measured against FP64, with the one-part rounding ~8e-5 on
+// top), an error the relaxed solve absorbs: products made once the residual
+// is small may carry errors up to about the target over the residual.
+//
+// Dofs of degree n are [n^2, (n + 1)^2); padding dofs (>= (P + 1)^2) are zero
+// in every table and expansion.
+constexpr int dof_degree(int i) {
+  int d = 0;
+  while ((d + 1) * (d + 1) <= i) ++d;
+  return d;
+}
+
+template <int KP, int P>
+struct DegreeCut {
+  static constexpr int NRB = KP / 16, NK = KP / 32, NW = KP / 32;
+  static constexpr int ND = (P + 1) * (P + 1);
+  static constexpr bool keep(int rb, int ks) {
+    return 16 * rb < ND && 32 * ks < ND && dof_degree(16 * rb) + dof_degree(32 * ks) <= P;
+  }
+  static constexpr int cost(int rb) {
+    int c = 0;
+    for (int k = 0; k < NK; ++k) c += keep(rb, k) ? 1 : 0;
+    return c;
+  }
+};
+
+// One wave's share: its row blocks and the (row block, k step) fragments it
+// multiplies, in k-step order.  Row blocks go to waves longest first, each to
+// the least loaded wave, so the waves' MFMA counts differ by at most one
+// block (P = 10, KP = 128: 4, 3, 3, 3 blocks against 8 per wave uncut).
+struct CutWave {
+  int nr;         // row blocks owned (written, zeros where no block is kept)
+  int rb[8];
+  int nf;         // fragments multiplied per offset
+  int f_row[32];  // index into rb[]
+  int f_ks[32];
+  bool ks_used[4];
+};
+
+template <int KP, int P>
+constexpr CutWave cut_wave(int w) {
+  using C = DegreeCut<KP, P>;
+  int owner[8] = {};
+  int load[4] = {};
+  bool placed[8] = {};
+  for (int step = 0; step < C::NRB; ++step) {
+    int best = -1;
+    for (int rb = 0; rb < C::NRB; ++rb)
+      if (!placed[rb] && (best < 0 || C::cost(rb) > C::cost(best))) best = rb;
+    int least = 0;
+    for (int v = 1; v < C::NW; ++v)
+      if (load[v] < load[least]) least = v;
+    placed[best] = true;
+    owner[best] = least;
+    load[least] += C::cost(best);
+  }
+  CutWave s{};
+  for (int rb = 0; rb < C::NRB; ++rb)
+    if (owner[rb] == w) s.rb[s.nr++] = rb;
+  for (int ks = 0; ks < C::NK; ++ks)
+    for (int i = 0; i < s.nr; ++i)
+      if (C::keep(s.rb[i], ks)) {
+        s.f_row[s.nf] = i;
+        s.f_ks[s.nf] = ks;
+        ++s.nf;
+        s.ks_used[ks] = true;
+      }
+  return s;
+}
+
+// The offset loop of wave W (the waves differ only in which fragments they
+// multiply).  Same tiles, slot lists, gather and pipeline as m2l_kernel
+// with NPART = 1; the wave's operator fragments for the next offset are
+// loaded whole while the current offset multiplies.
+template <int KP, int P, int W>
+__device__ __forceinline__ void m2l_cut_wave(
+    __amdgpu_buffer_rsrc_t msrc_rsrc, float *__restrict__ Lt, const uint16_t *__restrict__ Tb,
+    const int32_t *__restrict__ src_slots, int src_ld, int empty_idx,
+    const int32_t *__restrict__ smap, const int32_t *__restrict__ tl, int na, uint16_t *sB,
+    const int *s_tgt, int tid) {
+  constexpr CutWave S = cut_wave<KP, P>(W);
+  constexpr int NT = 2 * KP, LDB = KP + 8;
+  constexpr int NR = S.nr > 0 ? S.nr : 1, NF = S.nf > 0 ? S.nf : 1;
+  using GA = M2LGather<KP, 1>;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int lane = tid & 63, r16 = lane & 15, kq = lane >> 4;
+  f4 acc[NR][4];
+#pragma unroll
+  for (int i = 0; i < NR; ++i)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[i][cb] = f4{0.f, 0.f, 0.f, 0.f};
+  int tgt_of[GA::IT];
+#pragma unroll
+  for (int it = 0; it < GA::IT; ++it) tgt_of[it] = s_tgt[(tid + it * NT) / GA::PER_T];
+  int src[GA::IT];
+  float4 v[GA::IT];
+  auto src_load = [&](int slot, bool live) {
+#pragma unroll
+    for (int it = 0; it < GA::IT; ++it) {
+      const int row = live && tgt_of[it] >= 0 ? tgt_of[it] : empty_idx;
+      src[it] = src_slots[static_cast<int64_t>(row) * src_ld + slot];
+    }
+  };
+  auto table_of = [&](int pc) -> const uint16_t * {
+    return Tb + static_cast<int64_t>(smap[tl[pc]]) * 3 * KP * KP;  // part 0
+  };
+  auto a_load = [&](const uint16_t *__restrict__ A, bf16x8 (&dst)[NF]) {
+#pragma unroll
+    for (int f = 0; f < S.nf; ++f)
+      dst[f] = *reinterpret_cast<const bf16x8 *>(
+          A + ((static_cast<int64_t>(S.f_ks[f]) * (KP / 16) + S.rb[S.f_row[f]]) * 64 + lane) * 8);
+  };
+  // Operator fragments in two sets that swap roles every offset (the loop
+  // takes offsets in pairs): a copy from the prefetch set into the working
+  // set would make the compiler wait for the prefetch, and with it for every
+  // load issued before it, inside the iteration that issued it.  An odd count
+  // runs one more offset past the end, whose gathers read no sources (zeros).
+  bf16x8 fa[NF], fb[NF];
+  if (na > 0) {
+    src_load(tl[0], true);
+    m2l_mult_load<KP, 1>(msrc_rsrc, src, tid, v);
+    src_load(tl[min(1, na - 1)], na > 1);
+    a_load(table_of(0), fa);
+  }
+  auto offset_step = [&](int pc, const bf16x8 (&cur)[NF], bf16x8 (&nxt)[NF]) {
+    __syncthreads();  // the previous offset's multipoles are consumed
+#pragma unroll
+    for (int it = 0; it < GA::IT; ++it) {
+      const int e = tid + it * NT;
+      const int j = e / GA::PER_T, q8 = e % GA::PER_T;
+      *reinterpret_cast<float4 *>(&sB[j * LDB + 8 * q8]) = v[it];
+    }
+    __syncthreads();
+    m2l_mult_load<KP, 1>(msrc_rsrc, src, tid, v);
+    src_load(tl[min(pc + 2, na - 1)], pc + 2 < na);
+    a_load(table_of(min(pc + 1, na - 1)), nxt);
+    // the scheduler would otherwise move these prefetches next to their
+    // uses in the next step
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ks = 0; ks < DegreeCut<KP, P>::NK; ++ks) {
+      if (!S.ks_used[ks]) continue;
+      bf16x8 b[4];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+        b[cb] = *reinterpret_cast<const bf16x8 *>(&sB[(16 * cb + r16) * LDB + 32 * ks + 8 * kq]);
+#pragma unroll
+      for (int f = 0; f < S.nf; ++f) {
+        if (S.f_ks[f] != ks) continue;
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb)
+          acc[S.f_row[f]][cb] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[f], b[cb], acc[S.f_row[f]][cb], 0, 0, 0);
+      }
+    }
+  };
+  for (int pc = 0; pc < na; pc += 2) {
+    offset_step(pc, fa, fb);
+    offset_step(pc + 1, fb, fa);
+  }
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) {
+    const int tgt = s_tgt[r16 + 16 * cb];
+    if (tgt < 0) continue;
+#pragma unroll
+    for (int i = 0; i < S.nr; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Lt[static_cast<int64_t>(tgt) * KP + 16 * S.rb[i] + 4 * kq + r] = acc[i][cb][r];
+  }
+}
+
+template <int KP, int P>
+__global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(3)))
+void m2l_cut_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *__restrict__ Lt,
+                    const uint16_t *__restrict__ Tb, const M2LTile *__restrict__ tiles, int ntiles,
+                    const int32_t *__restrict__ sorted_boxes, const int32_t *__restrict__ src_slots,
+                    int src_ld, int empty_idx, const int32_t *__restrict__ slot_table, int slot_ld,
+                    const int32_t *__restrict__ tile_list, const int32_t *__restrict__ tile_nlist,
+                    int list_ld, SysMask mask) {
+  constexpr int NT = 2 * KP, LDB = KP + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t sB[kTileT * LDB];
+  __shared__ int s_tgt[kTileT];
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+  if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  const M2LTile tile = tiles[tile_id];
+  const int tid = threadIdx.x;
+  for (int j = tid; j < kTileT; j += NT) s_tgt[j] = j < tile.count ? sorted_boxes[tile.first + j] : -1;
+  __syncthreads();
+  if (mask.active) {
+    int live = 0;
+    for (int j = tid; j < tile.count; j += NT) live |= mask.active[mask.box_sys[s_tgt[j]]];
+    if (!__syncthreads_or(live)) return;  // uniform over the workgroup
+  }
+  const int32_t *__restrict__ smap = slot_table + static_cast<int64_t>(tile.slots) * slot_ld;
+  const int32_t *__restrict__ tl = tile_list + static_cast<int64_t>(tile_id) * list_ld;
+  const int na = tile_nlist[tile_id];
+  const __amdgpu_buffer_rsrc_t msrc_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void *>(Msrc), 0, static_cast<int>(msrc_bytes), kBufferRsrcWord3);
+  // the wave index as a scalar: each wave runs its own loop (no branch joins
+  // inside a loop, so the compiler's load waits stay per load)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+#define MF_CUT_ARGS msrc_rsrc, Lt, Tb, src_slots, src_ld, empty_idx, smap, tl, na, sB, s_tgt, tid
+  if (wave == 0) m2l_cut_wave<KP, P, 0>(MF_CUT_ARGS);
+  else if constexpr (KP / 32 > 1) {
+    if (wave == 1) m2l_cut_wave<KP, P, 1>(MF_CUT_ARGS);
+    else if constexpr (KP / 32 > 2) {
+      if (wave == 2) m2l_cut_wave<KP, P, 2>(MF_CUT_ARGS);
+      else if constexpr (KP / 32 > 3) m2l_cut_wave<KP, P, 3>(MF_CUT_ARGS);
+    }
+  }
+#undef MF_CUT_ARGS
+}
+
 // Non-empty offsets of every M2L tile: list[tile][0 .. nlist[tile]) = the
 // slots (in order) where at least one target of the tile has a source box.
 // One workgroup per tile; nslot <= 1024.
@@ -2607,7 +2831,7 @@ Status fmm_info(mf_plan *plan, mf_plan_info *info) {
 
 namespace {
 
-template <int KP>
+template <int P, int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
                   int ntiles, const int32_t *src, int src_rows, int nslot, const int32_t *tlist,
                   const int32_t *tn, SysMask mask, bool fence) {
@@ -2651,6 +2875,12 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
       else hipLaunchKernelGGL((m2l_kernel<KP, 1>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
+    case MF_M2L_BF16X1_CUT:
+      hipLaunchKernelGGL((m2l_cut_kernel<KP, P>), grid, block, 0, ctx->stream, F.Mb.get(),
+                         msrc_bytes, F.Lt.as<float>(), tab.Tb.as<uint16_t>(), tiles, ntiles,
+                         F.class_ids2.as<int32_t>(), src, nslot, src_rows,
+                         tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, mask);
+      break;
     default:
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
   }
@@ -2744,20 +2974,20 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &b
     const bool own = plan->m2l_precision() == plan->opts.m2l_precision;
     KernelTimer timer(ctx, hess ? "m2l_force" : own ? "m2l" : "m2l_relaxed");
     if (!hess) {
-      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
+      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
                             F.nbox, tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>(), mask,
-                            false));
+                            false)));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
       // force lists srcH (indexed by leaf = global index), the others srcF.
       const int lt = F.leaf_tile_start;
-      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
+      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
                             F.nbox, tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>(), mask,
-                            true));
-      MF_TRY(launch_m2l<KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
+                            true)));
+      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
                             F.srcH.as<int32_t>(), F.count[D], tab.nslotH,
                             F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * tab.slot_ld,
-                            F.tnH.as<int32_t>() + lt, mask, true));
+                            F.tnH.as<int32_t>() + lt, mask, true)));
     }
   }
   {

@@ -42,6 +42,7 @@ M2L_FP32 = 0
 M2L_BF16X6 = 1
 M2L_BF16X3 = 2
 M2L_BF16X1 = 3
+M2L_BF16X1_CUT = 4  # BF16X1 cut to the translation terms of low total degree (relaxed products)
 SOLVE_CG = 0
 SOLVE_GMRES = 1
 SOLVE_REF_NLCG = 2

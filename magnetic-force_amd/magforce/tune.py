@@ -17,11 +17,11 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import (M2L_BF16X1, M2L_BF16X3, M2L_BF16X6, M2L_FP32, MODE_FMM, Context, Plan,
-               field_allpairs_f64)
+from . import (M2L_BF16X1, M2L_BF16X1_CUT, M2L_BF16X3, M2L_BF16X6, M2L_FP32, MODE_FMM,
+               Context, Plan, field_allpairs_f64)
 
 PRECISION_NAMES = {M2L_FP32: "fp32", M2L_BF16X6: "bf16x6", M2L_BF16X3: "bf16x3",
-                   M2L_BF16X1: "bf16x1"}
+                   M2L_BF16X1: "bf16x1", M2L_BF16X1_CUT: "bf16x1cut"}
 
 
 @dataclass

@@ -101,12 +101,12 @@ def test_batch_solve_per_system(ctx, oracle, batch_case, m2l):
         assert rel_err(x[a:e], x_ref) < 5e-6, s
 
 
-@pytest.mark.parametrize("graph_iters", [0, 2])
-def test_batch_relaxed_products(ctx, oracle, batch_case, graph_iters):
-    """Batched CG with relaxed products (one-part BF16 translations once the
-    largest relative residual is at 3e-3): every system still meets 1e-6 and
-    its oracle solution; with graph replay the switch re-captures the
-    iteration."""
+@pytest.mark.parametrize("graph_iters,relax", [(0, "bf16x1"), (2, "bf16x1"), (2, "bf16x1cut")])
+def test_batch_relaxed_products(ctx, oracle, batch_case, graph_iters, relax):
+    """Batched CG with relaxed products (one-part BF16 translations, or
+    their degree cut, once the largest relative residual is at 3e-3): every
+    system still meets 1e-6 and its oracle solution; with graph replay the
+    switch re-captures the iteration."""
     import magforce
     systems, off, pos, _ = batch_case
     rng = np.random.default_rng(18)
@@ -117,8 +117,10 @@ def test_batch_relaxed_products(ctx, oracle, batch_case, graph_iters):
     b = np.concatenate([np.tile(hdir[s], (len(p), 1)) for s, p in enumerate(systems)])
     plan = magforce.BatchPlan(ctx, dev(pos), off, 2.0, m2l_precision=magforce.M2L_BF16X3)
     res = plan.solve(kappas, dev(b), rtol=1e-6, graph_iters=graph_iters,
-                     relax_m2l_precision=magforce.M2L_BF16X1, relax_rtol=3e-3)
-    print(f"batch relaxed (graph_iters={graph_iters}): {res.stats['matvecs']} products, "
+                     relax_m2l_precision={"bf16x1": magforce.M2L_BF16X1,
+                                          "bf16x1cut": magforce.M2L_BF16X1_CUT}[relax],
+                     relax_rtol=3e-3)
+    print(f"batch relaxed ({relax}, graph_iters={graph_iters}): {res.stats['matvecs']} products, "
           f"{res.stats['relaxed_matvecs']} relaxed")
     assert res.stats["converged"] == 1
     assert 1 <= res.stats["relaxed_matvecs"] < res.stats["matvecs"]

@@ -82,6 +82,31 @@ def test_fmm_bf16_m2l_matches_oracle(ctx, slab_case, order, prec):
         assert rel_err(F, F32) < 2e-7
 
 
+@pytest.mark.parametrize("order", [6, 8, 10])
+def test_fmm_bf16x1_cut_matches_oracle(ctx, slab_case, order):
+    """Degree-cut one-part translations (the relaxed products' M2L): the cut
+    drops the MFMA blocks whose every term has local plus multipole degree
+    above the order, terms that fall off like rho^(j + n).  Field and force
+    stay within a few 1e-4 of the oracle and near the uncut one-part product;
+    a wrongly dropped low-degree block would put them off by 1e-2 or more,
+    and a cut that dropped nothing would leave them equal to it."""
+    import magforce
+    pos, m, field, force = slab_case
+    cut = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order,
+                        m2l_precision=magforce.M2L_BF16X1_CUT)
+    one = magforce.Plan(ctx, dev(pos), 2.0, mode=magforce.MODE_FMM, order=order,
+                        m2l_precision=magforce.M2L_BF16X1)
+    mm = dev(m)
+    g, g1 = host(cut.field(mm)), host(one.field(mm))
+    F, F1 = host(cut.force(mm)), host(one.force(mm))
+    eg, ef = rel_err(g, field), rel_err(F, force)
+    dg, df = rel_err(g, g1), rel_err(F, F1)
+    print(f"order {order}: cut field {eg:.3e} force {ef:.3e}; uncut field "
+          f"{rel_err(g1, field):.3e} force {rel_err(F1, force):.3e}; cut vs uncut {dg:.3e} {df:.3e}")
+    assert eg < 3e-4 and ef < 3e-4
+    assert 0.0 < dg < 2e-4 and 0.0 < df < 2e-4
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (2, 2, 2), (3, 7, 2), (9, 9, 9)])
 def test_fmm_small_and_shallow_trees(ctx, oracle, shape):
     """Trees of one or two levels have no far field (exact near field only);

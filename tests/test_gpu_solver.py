@@ -132,16 +132,19 @@ def test_fmm_refinement_vs_oracle(ctx, oracle, outer):
     assert rel_err(F, F_ref) < 1e-5
 
 
-def test_relaxed_cg_vs_oracle(ctx, oracle):
+@pytest.mark.parametrize("relax", ["bf16x1", "bf16x1cut"])
+def test_relaxed_cg_vs_oracle(ctx, oracle, relax):
     """CG with relaxed products: once the relative residual falls to 3e-3 the
     remaining products run with one-part BF16 translations (a few 1e-4 of
-    the field).  An error e made at residual r moves the final residual by
-    about e r, so the solution stays on the BF16x3 solve's accuracy (within
-    the p = 10 truncation of the FP64 oracle), and the force with it.  A
-    relax threshold at the tolerance itself relaxes nothing, and a plan
-    without a far field (all-pairs) ignores the option."""
+    the field), or with their degree cut.  An error e made at residual r
+    moves the final residual by about e r, so the solution stays on the
+    BF16x3 solve's accuracy (within the p = 10 truncation of the FP64
+    oracle), and the force with it.  A relax threshold at the tolerance
+    itself relaxes nothing, and a plan without a far field (all-pairs)
+    ignores the option."""
     import magforce
     from magforce import configs
+    code = {"bf16x1": magforce.M2L_BF16X1, "bf16x1cut": magforce.M2L_BF16X1_CUT}[relax]
     s = configs.lattice8k()
     pos = s.pos.astype(np.float64)
     b = np.tile(s.hconst, (s.n, 1))
@@ -151,10 +154,10 @@ def test_relaxed_cg_vs_oracle(ctx, oracle):
                          m2l_precision=magforce.M2L_BF16X3)
     full = plan.solve(s.kappa, dev(b), rtol=1e-6)
     g_before = host(plan.field(full.x))
-    res = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1,
+    res = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=code,
                      relax_rtol=3e-3, true_residual=True)
     st = res.stats
-    print(f"relaxed cg: {st} (plain: {full.stats})")
+    print(f"relaxed cg ({relax}): {st} (plain: {full.stats})")
     assert st["converged"] == 1 and st["relaxed_matvecs"] >= 1
     assert st["relaxed_matvecs"] < st["matvecs"]
     assert st["rel_residual_true"] <= 2e-6
@@ -165,12 +168,12 @@ def test_relaxed_cg_vs_oracle(ctx, oracle):
     assert rel_err(F, F_ref) < 1e-5
     # the plan's own precision is back after the solve
     assert rel_err(host(plan.field(full.x)), g_before) == 0.0
-    none = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1,
+    none = plan.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=code,
                       relax_rtol=1e-7)
     assert none.stats["relaxed_matvecs"] == 0
     assert rel_err(host(none.x), host(full.x)) == 0.0
     ap = magforce.Plan(ctx, dev(pos), s.h)
-    res_ap = ap.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1)
+    res_ap = ap.solve(s.kappa, dev(b), rtol=1e-6, relax_m2l_precision=code)
     assert res_ap.stats["relaxed_matvecs"] == 0
 
 
