@@ -763,18 +763,23 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
 }
 
 // ---------------------------------------------------------------------------
-// Degree-cut one-part M2L (MF_M2L_BF16X1_CUT), for the late products of a
-// relaxed solve.  The translation L_j = sum_n T_jn M_n couples local degree j
-// with multipole degree n; its terms fall off like rho^(j + n) (rho = box
-// radius over separation, below 1/2 for the interaction lists here).  This
-// kernel keeps the 16 x 32 MFMA blocks (row block rb: local dofs 16 rb ..,
-// k step ks: multipole dofs 32 ks ..) whose lowest degrees add up to at most
-// P -- for P = 10, 13 of the 32 blocks -- and drops the rest, whose terms all
-// have j + n > P.  Dropped, they change a product by ~4e-5 relative on the
-// 1.15 M pool (measured against FP64, with the one-part rounding ~8e-5 on
-// top), an error the relaxed solve absorbs: products made once the residual
-// is small may carry errors up to about the target over the residual.
-//
+// Degree-ordered M2L.  The translation L_j = sum_n T_jn M_n couples local
+// degree j with multipole degree n; its terms fall off like rho^(j + n)
+// (rho = box radius over separation, below 1/2 for the interaction lists
+// here).  These kernels sort the 16 x 32 MFMA blocks (row block rb: local
+// dofs 16 rb .. 16 rb + 15; k step ks: multipole dofs 32 ks .. 32 ks + 31)
+// by the lowest degrees in them: a block is "low" when those add up to at
+// most P.  At P = 10, 13 of the 32 blocks are low; every term of the other
+// 19 has j + n > P, and together they are ~4e-5 of a product on the 1.15 M
+// pool (measured against FP64).
+//  * MF_M2L_BF16X3 (SPLIT): the low blocks take the three-product split of
+//    m2l_kernel, the others one product of the leading BF16 parts (their
+//    ~4e-3 rounding of terms that small adds ~2e-7): the split's accuracy
+//    from 58 instead of 96 MFMAs per offset and column block, and 45
+//    instead of 64 KB of operator fragments.
+//  * MF_M2L_BF16X1_CUT (!SPLIT): the low blocks in one product, the others
+//    dropped: ~1e-4 relative per product, for the late products of a
+//    relaxed solve, whose errors may grow as the residual falls.
 // Dofs of degree n are [n^2, (n + 1)^2); padding dofs (>= (P + 1)^2) are zero
 // in every table and expansion.
 constexpr int dof_degree(int i) {
@@ -784,77 +789,94 @@ constexpr int dof_degree(int i) {
 }
 
 template <int KP, int P>
-struct DegreeCut {
+struct DegreeBlocks {
   static constexpr int NRB = KP / 16, NK = KP / 32, NW = KP / 32;
   static constexpr int ND = (P + 1) * (P + 1);
-  static constexpr bool keep(int rb, int ks) {
+  static constexpr bool low(int rb, int ks) {
     return 16 * rb < ND && 32 * ks < ND && dof_degree(16 * rb) + dof_degree(32 * ks) <= P;
   }
-  static constexpr int cost(int rb) {
+  // MFMA products of row block rb per column block
+  static constexpr int cost(int rb, bool split) {
     int c = 0;
-    for (int k = 0; k < NK; ++k) c += keep(rb, k) ? 1 : 0;
+    for (int k = 0; k < NK; ++k) c += low(rb, k) ? (split ? 3 : 1) : (split ? 1 : 0);
     return c;
   }
 };
 
-// One wave's share: its row blocks and the (row block, k step) fragments it
-// multiplies, in k-step order.  Row blocks go to waves longest first, each to
-// the least loaded wave, so the waves' MFMA counts differ by at most one
-// block (P = 10, KP = 128: 4, 3, 3, 3 blocks against 8 per wave uncut).
-struct CutWave {
-  int nr;         // row blocks owned (written, zeros where no block is kept)
+// One wave's share: its row blocks, the blocks it multiplies (k-step order)
+// and the operator fragments it loads.  Row blocks go to waves longest
+// first, each to the least loaded wave (P = 10, KP = 128: 18, 14, 14, 12
+// products per column block with SPLIT, against 24 per wave for m2l_kernel;
+// 4, 3, 3, 3 for the cut, against 8).
+struct DegWave {
+  int nr;  // row blocks owned (all written; zeros where nothing is multiplied)
   int rb[8];
-  int nf;         // fragments multiplied per offset
-  int f_row[32];  // index into rb[]
-  int f_ks[32];
-  bool ks_used[4];
+  int nb;  // blocks
+  int b_row[32], b_ks[32], b_f0[32], b_f1[32];  // b_f1 = -1: one product
+  int nf;  // operator fragments (16 x 32 BF16, one part)
+  int f_rb[64], f_ks[64], f_part[64];
+  bool ks_used[4], ks_part1[4];  // k steps read from LDS, and with part 1
 };
 
-template <int KP, int P>
-constexpr CutWave cut_wave(int w) {
-  using C = DegreeCut<KP, P>;
+template <int KP, int P, bool SPLIT>
+constexpr DegWave deg_wave(int w) {
+  using C = DegreeBlocks<KP, P>;
   int owner[8] = {};
   int load[4] = {};
   bool placed[8] = {};
   for (int step = 0; step < C::NRB; ++step) {
     int best = -1;
     for (int rb = 0; rb < C::NRB; ++rb)
-      if (!placed[rb] && (best < 0 || C::cost(rb) > C::cost(best))) best = rb;
+      if (!placed[rb] && (best < 0 || C::cost(rb, SPLIT) > C::cost(best, SPLIT))) best = rb;
     int least = 0;
     for (int v = 1; v < C::NW; ++v)
       if (load[v] < load[least]) least = v;
     placed[best] = true;
     owner[best] = least;
-    load[least] += C::cost(best);
+    load[least] += C::cost(best, SPLIT);
   }
-  CutWave s{};
+  DegWave s{};
   for (int rb = 0; rb < C::NRB; ++rb)
     if (owner[rb] == w) s.rb[s.nr++] = rb;
   for (int ks = 0; ks < C::NK; ++ks)
-    for (int i = 0; i < s.nr; ++i)
-      if (C::keep(s.rb[i], ks)) {
-        s.f_row[s.nf] = i;
+    for (int i = 0; i < s.nr; ++i) {
+      const bool lo = C::low(s.rb[i], ks);
+      if (!lo && !SPLIT) continue;
+      const int b = s.nb++;
+      s.b_row[b] = i;
+      s.b_ks[b] = ks;
+      s.f_rb[s.nf] = s.rb[i];
+      s.f_ks[s.nf] = ks;
+      s.f_part[s.nf] = 0;
+      s.b_f0[b] = s.nf++;
+      s.b_f1[b] = -1;
+      if (lo && SPLIT) {
+        s.f_rb[s.nf] = s.rb[i];
         s.f_ks[s.nf] = ks;
-        ++s.nf;
-        s.ks_used[ks] = true;
+        s.f_part[s.nf] = 1;
+        s.b_f1[b] = s.nf++;
+        s.ks_part1[ks] = true;
       }
+      s.ks_used[ks] = true;
+    }
   return s;
 }
 
 // The offset loop of wave W (the waves differ only in which fragments they
-// multiply).  Same tiles, slot lists, gather and pipeline as m2l_kernel
-// with NPART = 1; the wave's operator fragments for the next offset are
-// loaded whole while the current offset multiplies.
-template <int KP, int P, int W>
-__device__ __forceinline__ void m2l_cut_wave(
+// load and multiply).  Same tiles, slot lists, gather and pipeline as
+// m2l_kernel; the wave's operator fragments for the next offset are loaded
+// whole while the current offset multiplies.
+template <int KP, int P, bool SPLIT, int W>
+__device__ __forceinline__ void m2l_deg_wave(
     __amdgpu_buffer_rsrc_t msrc_rsrc, float *__restrict__ Lt, const uint16_t *__restrict__ Tb,
     const int32_t *__restrict__ src_slots, int src_ld, int empty_idx,
     const int32_t *__restrict__ smap, const int32_t *__restrict__ tl, int na, uint16_t *sB,
     const int *s_tgt, int tid) {
-  constexpr CutWave S = cut_wave<KP, P>(W);
-  constexpr int NT = 2 * KP, LDB = KP + 8;
+  constexpr DegWave S = deg_wave<KP, P, SPLIT>(W);
+  constexpr int NPG = SPLIT ? 2 : 1;  // multipole parts gathered
+  constexpr int NT = 2 * KP, LDB = SPLIT ? KP + 24 : KP + 8;
   constexpr int NR = S.nr > 0 ? S.nr : 1, NF = S.nf > 0 ? S.nf : 1;
-  using GA = M2LGather<KP, 1>;
+  using GA = M2LGather<KP, NPG>;
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int lane = tid & 63, r16 = lane & 15, kq = lane >> 4;
   f4 acc[NR][4];
@@ -875,13 +897,14 @@ __device__ __forceinline__ void m2l_cut_wave(
     }
   };
   auto table_of = [&](int pc) -> const uint16_t * {
-    return Tb + static_cast<int64_t>(smap[tl[pc]]) * 3 * KP * KP;  // part 0
+    return Tb + static_cast<int64_t>(smap[tl[pc]]) * 3 * KP * KP;
   };
   auto a_load = [&](const uint16_t *__restrict__ A, bf16x8 (&dst)[NF]) {
 #pragma unroll
     for (int f = 0; f < S.nf; ++f)
       dst[f] = *reinterpret_cast<const bf16x8 *>(
-          A + ((static_cast<int64_t>(S.f_ks[f]) * (KP / 16) + S.rb[S.f_row[f]]) * 64 + lane) * 8);
+          A + (((static_cast<int64_t>(S.f_part[f]) * (KP / 32) + S.f_ks[f]) * (KP / 16) + S.f_rb[f]) *
+                   64 + lane) * 8);
   };
   // Operator fragments in two sets that swap roles every offset (the loop
   // takes offsets in pairs): a copy from the prefetch set into the working
@@ -891,7 +914,7 @@ __device__ __forceinline__ void m2l_cut_wave(
   bf16x8 fa[NF], fb[NF];
   if (na > 0) {
     src_load(tl[0], true);
-    m2l_mult_load<KP, 1>(msrc_rsrc, src, tid, v);
+    m2l_mult_load<KP, NPG>(msrc_rsrc, src, tid, v);
     src_load(tl[min(1, na - 1)], na > 1);
     a_load(table_of(0), fa);
   }
@@ -900,30 +923,40 @@ __device__ __forceinline__ void m2l_cut_wave(
 #pragma unroll
     for (int it = 0; it < GA::IT; ++it) {
       const int e = tid + it * NT;
-      const int j = e / GA::PER_T, q8 = e % GA::PER_T;
-      *reinterpret_cast<float4 *>(&sB[j * LDB + 8 * q8]) = v[it];
+      const int j = e / GA::PER_T, q = e % GA::PER_T;
+      const int p = q / (KP / 8), q8 = q % (KP / 8);
+      *reinterpret_cast<float4 *>(&sB[(p * kTileT + j) * LDB + 8 * q8]) = v[it];
     }
     __syncthreads();
-    m2l_mult_load<KP, 1>(msrc_rsrc, src, tid, v);
+    m2l_mult_load<KP, NPG>(msrc_rsrc, src, tid, v);
     src_load(tl[min(pc + 2, na - 1)], pc + 2 < na);
     a_load(table_of(min(pc + 1, na - 1)), nxt);
     // the scheduler would otherwise move these prefetches next to their
     // uses in the next step
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int ks = 0; ks < DegreeCut<KP, P>::NK; ++ks) {
+    for (int ks = 0; ks < DegreeBlocks<KP, P>::NK; ++ks) {
       if (!S.ks_used[ks]) continue;
-      bf16x8 b[4];
+      bf16x8 b0[4], b1[4];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
-        b[cb] = *reinterpret_cast<const bf16x8 *>(&sB[(16 * cb + r16) * LDB + 32 * ks + 8 * kq]);
+      for (int cb = 0; cb < 4; ++cb) {
+        b0[cb] = *reinterpret_cast<const bf16x8 *>(&sB[(16 * cb + r16) * LDB + 32 * ks + 8 * kq]);
+        if (S.ks_part1[ks])
+          b1[cb] = *reinterpret_cast<const bf16x8 *>(
+              &sB[(kTileT + 16 * cb + r16) * LDB + 32 * ks + 8 * kq]);
+      }
 #pragma unroll
-      for (int f = 0; f < S.nf; ++f) {
-        if (S.f_ks[f] != ks) continue;
+      for (int b = 0; b < S.nb; ++b) {
+        if (S.b_ks[b] != ks) continue;
+        const int i = S.b_row[b];
 #pragma unroll
-        for (int cb = 0; cb < 4; ++cb)
-          acc[S.f_row[f]][cb] =
-              __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[f], b[cb], acc[S.f_row[f]][cb], 0, 0, 0);
+        for (int cb = 0; cb < 4; ++cb) {
+          if (S.b_f1[b] >= 0) {
+            acc[i][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[S.b_f0[b]], b1[cb], acc[i][cb], 0, 0, 0);
+            acc[i][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[S.b_f1[b]], b0[cb], acc[i][cb], 0, 0, 0);
+          }
+          acc[i][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(cur[S.b_f0[b]], b0[cb], acc[i][cb], 0, 0, 0);
+        }
       }
     }
   };
@@ -943,16 +976,18 @@ __device__ __forceinline__ void m2l_cut_wave(
   }
 }
 
-template <int KP, int P>
-__global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(3)))
-void m2l_cut_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *__restrict__ Lt,
+// Two waves per SIMD for the split (two operator sets of up to 13
+// fragments), three for the cut.
+template <int KP, int P, bool SPLIT>
+__global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(SPLIT ? 2 : 3)))
+void m2l_deg_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *__restrict__ Lt,
                     const uint16_t *__restrict__ Tb, const M2LTile *__restrict__ tiles, int ntiles,
                     const int32_t *__restrict__ sorted_boxes, const int32_t *__restrict__ src_slots,
                     int src_ld, int empty_idx, const int32_t *__restrict__ slot_table, int slot_ld,
                     const int32_t *__restrict__ tile_list, const int32_t *__restrict__ tile_nlist,
                     int list_ld, SysMask mask) {
-  constexpr int NT = 2 * KP, LDB = KP + 8;
-  __shared__ __attribute__((aligned(16))) uint16_t sB[kTileT * LDB];
+  constexpr int NT = 2 * KP, LDB = SPLIT ? KP + 24 : KP + 8;
+  __shared__ __attribute__((aligned(16))) uint16_t sB[(SPLIT ? 2 : 1) * kTileT * LDB];
   __shared__ int s_tgt[kTileT];
   const int per_xcd = (ntiles + 7) / 8;
   const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
@@ -974,16 +1009,16 @@ void m2l_cut_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *_
   // the wave index as a scalar: each wave runs its own loop (no branch joins
   // inside a loop, so the compiler's load waits stay per load)
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-#define MF_CUT_ARGS msrc_rsrc, Lt, Tb, src_slots, src_ld, empty_idx, smap, tl, na, sB, s_tgt, tid
-  if (wave == 0) m2l_cut_wave<KP, P, 0>(MF_CUT_ARGS);
+#define MF_DEG_ARGS msrc_rsrc, Lt, Tb, src_slots, src_ld, empty_idx, smap, tl, na, sB, s_tgt, tid
+  if (wave == 0) m2l_deg_wave<KP, P, SPLIT, 0>(MF_DEG_ARGS);
   else if constexpr (KP / 32 > 1) {
-    if (wave == 1) m2l_cut_wave<KP, P, 1>(MF_CUT_ARGS);
+    if (wave == 1) m2l_deg_wave<KP, P, SPLIT, 1>(MF_DEG_ARGS);
     else if constexpr (KP / 32 > 2) {
-      if (wave == 2) m2l_cut_wave<KP, P, 2>(MF_CUT_ARGS);
-      else if constexpr (KP / 32 > 3) m2l_cut_wave<KP, P, 3>(MF_CUT_ARGS);
+      if (wave == 2) m2l_deg_wave<KP, P, SPLIT, 2>(MF_DEG_ARGS);
+      else if constexpr (KP / 32 > 3) m2l_deg_wave<KP, P, SPLIT, 3>(MF_DEG_ARGS);
     }
   }
-#undef MF_CUT_ARGS
+#undef MF_DEG_ARGS
 }
 
 // Non-empty offsets of every M2L tile: list[tile][0 .. nlist[tile]) = the
@@ -2846,6 +2881,10 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
       F.class_ids2.as<int32_t>(), src, nslot, src_rows, tab.slot_tab.as<int32_t>(),            \
       tab.slot_ld, tlist,                                                                      \
       tn, tab.slot_ld, 0, mask
+#define MF_DEG_LAUNCH                                                                         \
+  ctx->stream, F.Mb.get(), msrc_bytes, F.Lt.as<float>(), tab.Tb.as<uint16_t>(), tiles, ntiles, \
+      F.class_ids2.as<int32_t>(), src, nslot, src_rows, tab.slot_tab.as<int32_t>(),            \
+      tab.slot_ld, tlist, tn, tab.slot_ld, mask
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
@@ -2861,13 +2900,9 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
       else hipLaunchKernelGGL((m2l_kernel<KP, 3>), grid, block, 0, MF_M2L_ARGS);
       break;
     }
-    case MF_M2L_BF16X3: {
-      const void *tables = tab.Tb.get();
-      const void *msrc = F.Mb.get();
-      if (fence) hipLaunchKernelGGL((m2l_kernel<KP, 2, false, true>), grid, block, 0, MF_M2L_ARGS);
-      else hipLaunchKernelGGL((m2l_kernel<KP, 2>), grid, block, 0, MF_M2L_ARGS);
+    case MF_M2L_BF16X3:
+      hipLaunchKernelGGL((m2l_deg_kernel<KP, P, true>), grid, block, 0, MF_DEG_LAUNCH);
       break;
-    }
     case MF_M2L_BF16X1: {
       const void *tables = tab.Tb.get();
       const void *msrc = F.Mb.get();
@@ -2876,15 +2911,13 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
       break;
     }
     case MF_M2L_BF16X1_CUT:
-      hipLaunchKernelGGL((m2l_cut_kernel<KP, P>), grid, block, 0, ctx->stream, F.Mb.get(),
-                         msrc_bytes, F.Lt.as<float>(), tab.Tb.as<uint16_t>(), tiles, ntiles,
-                         F.class_ids2.as<int32_t>(), src, nslot, src_rows,
-                         tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, mask);
+      hipLaunchKernelGGL((m2l_deg_kernel<KP, P, false>), grid, block, 0, MF_DEG_LAUNCH);
       break;
     default:
       return Status::Err(MF_ERR_INVALID_ARG, "unknown M2L precision");
   }
 #undef MF_M2L_ARGS
+#undef MF_DEG_LAUNCH
   MF_LAUNCH_CHECK();
   return Status::Ok();
 }
