@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -90,6 +91,25 @@ def parse():
                    help="JSON with measured HBM bytes per launch of the dominant kernel "
                         "(from the rocprofv3 counter runs under profiles/)")
     return p.parse_args()
+
+
+def m2l_issued_per_tile_slot(order, m2l):
+    """MFMA FLOPs the field M2L issues per tile-slot (64 target boxes, one
+    offset).  FP32 and BF16x6: the whole KP x KP operator, 1 or 6 products.
+    BF16x3 (degree-split kernel): per 16 x 32 block of the operator, 3
+    products where the lowest local and multipole degrees in the block add up
+    to at most the order, 1 product elsewhere, none on pure padding."""
+    nd = (order + 1) ** 2
+    kp = (nd + 31) // 32 * 32
+    if m2l != "bf16x3":
+        return 64.0 * kp * kp * 2 * {"fp32": 1, "bf16x6": 6}[m2l]
+    prods = 0
+    for rb in range(kp // 16):
+        for ks in range(kp // 32):
+            if 16 * rb >= nd or 32 * ks >= nd:
+                continue
+            prods += 3 if math.isqrt(16 * rb) + math.isqrt(32 * ks) <= order else 1
+    return float(prods) * 4 * (16 * 16 * 32 * 2)  # 4 column blocks of 16 targets
 
 
 def relax_kwargs(args):
@@ -693,11 +713,10 @@ def main():
         # bytes per launch are reported beside it.
         order = info["order"]
         kp = ((order + 1) ** 2 + 31) // 32 * 32
-        nprod = {"fp32": 1, "bf16x6": 6, "bf16x3": 3}[args.m2l]
         m2l_flops = float(info["m2l_field_pairs"]) * 2.0 * (order + 1) ** 4
         m2l_peak = FP32_PEAK_TFLOPS if args.m2l == "fp32" else BF16_PEAK_TFLOPS
         m2l = kernel_roofline("m2l", m2l_flops, m2l_peak, "mfma")
-        issued = float(info["m2l_field_tile_slots"]) * 64 * kp * kp * 2 * nprod
+        issued = float(info["m2l_field_tile_slots"]) * m2l_issued_per_tile_slot(order, args.m2l)
         m2l["issued_mfma_flops_per_launch"] = issued
         m2l["issued_mfma_frac"] = (issued / (m2l["avg_launch_ms"] * 1e-3) / 1e12 / m2l_peak
                                    if m2l["avg_launch_ms"] > 0 else 0.0)

@@ -792,13 +792,15 @@ template <int KP, int P>
 struct DegreeBlocks {
   static constexpr int NRB = KP / 16, NK = KP / 32, NW = KP / 32;
   static constexpr int ND = (P + 1) * (P + 1);
+  // blocks that hold real dofs (the others are zero padding, skipped)
+  static constexpr bool real(int rb, int ks) { return 16 * rb < ND && 32 * ks < ND; }
   static constexpr bool low(int rb, int ks) {
-    return 16 * rb < ND && 32 * ks < ND && dof_degree(16 * rb) + dof_degree(32 * ks) <= P;
+    return real(rb, ks) && dof_degree(16 * rb) + dof_degree(32 * ks) <= P;
   }
   // MFMA products of row block rb per column block
   static constexpr int cost(int rb, bool split) {
     int c = 0;
-    for (int k = 0; k < NK; ++k) c += low(rb, k) ? (split ? 3 : 1) : (split ? 1 : 0);
+    for (int k = 0; k < NK; ++k) c += low(rb, k) ? (split ? 3 : 1) : (split && real(rb, k) ? 1 : 0);
     return c;
   }
 };
@@ -841,7 +843,7 @@ constexpr DegWave deg_wave(int w) {
   for (int ks = 0; ks < C::NK; ++ks)
     for (int i = 0; i < s.nr; ++i) {
       const bool lo = C::low(s.rb[i], ks);
-      if (!lo && !SPLIT) continue;
+      if (!lo && !(SPLIT && C::real(s.rb[i], ks))) continue;
       const int b = s.nb++;
       s.b_row[b] = i;
       s.b_ks[b] = ks;
