@@ -689,6 +689,33 @@ static void shift_quartic(const double *desc, double q0, double *asc_out) {
   }
 }
 
+void make_hess_table(double h, float *tab) {
+  const double scale = 1.0 / (3.0 * kMu0 / (4.0 * kPi) * h * h * h * h);
+  constexpr int per_unit = kHessTabN / 4;
+  double t[4];
+  for (int i = 0; i < 4; ++i) t[i] = -0.5 + 0.5 * std::cos((2 * i + 1) * kPi / 8.0);
+  for (int k = 0; k < kHessTabN; ++k) {
+    const int piece = k / per_unit;
+    double yc[4], yd[4], cc[4], cd[4];
+    for (int i = 0; i < 4; ++i) {
+      const double q = (k + 1 + t[i]) / per_unit;
+      double c1 = 0.0, c2 = 0.0;
+      for (int j = 0; j < 5; ++j) {
+        c1 = c1 * q + kC1[piece][j];
+        c2 = c2 * q + kC2[piece][j];
+      }
+      yc[i] = scale * c1;
+      yd[i] = -scale * (c2 - 3.0 * c1) / 5.0;
+    }
+    cubic_through(t, yc, cc);
+    cubic_through(t, yd, cd);
+    for (int j = 0; j < 4; ++j) {
+      tab[8 * k + j] = static_cast<float>(cc[j]);
+      tab[8 * k + 4 + j] = static_cast<float>(cd[j]);
+    }
+  }
+}
+
 const double (*reference_c1_table())[5] { return kC1; }
 const double (*reference_c2_table())[5] { return kC2; }
 
@@ -697,6 +724,7 @@ HessConsts make_hess_consts(double h) {
   hc.h = static_cast<float>(h);
   hc.inv_h = static_cast<float>(1.0 / h);
   hc.near2 = static_cast<float>(16.0 * h * h);
+  hc.tab_scale = static_cast<float>(kHessTabN / (4.0 * h));
   const double s = 3.0 * kMu0 / (4.0 * kPi);
   const double scale = 1.0 / (s * h * h * h * h);
   for (int k = 0; k < 4; ++k) {

@@ -1555,12 +1555,18 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
 // Force near-field sums over one contiguous source range, every STRIDE-th
 // source from j0.  NEAR: the range may hold pairs within 4h (fitted curves
 // below q = 4); otherwise every pair takes the far form, with no test.
+// The pair in the form with the far field's -5 folded away: with n5 = -5 n
+// and Rq = R / -5 (far: P ir^2; near: the table's second curve),
+//   e = Rq (d.m)(d.n5) + P (m.n) = R (d.m)(d.n) + P (m.n),
+// and the m-sums collect P (d.n5) = -5 P (d.n): the caller scales them by
+// -1/5.  One multiply per pair fewer than R = -5 P ir^2.
 template <int STRIDE, bool NEAR>
 __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int j0, int j1,
                                             float tx, float ty, float tz, float nx, float ny,
-                                            float nz, const HessConsts &hc, const float *tab,
-                                            float &dx0, float &dy0, float &dz0, float &mx0,
-                                            float &my0, float &mz0, float &n0) {
+                                            float nz, float nx5, float ny5, float nz5,
+                                            const HessConsts &hc, const float4 *tab, float &dx0,
+                                            float &dy0, float &dz0, float &mx0, float &my0,
+                                            float &mz0, float &n0) {
 #pragma unroll 2
   for (int j = j0; j < j1; j += STRIDE) {
     const float sx = rec[j].x, sy = rec[j].y, sz = rec[j].z;
@@ -1570,21 +1576,21 @@ __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int 
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float ir2 = ir * ir;
     float Pc = ir2 * ir2 * ir;
-    float Rc = -5.f * Pc * ir2;
+    float Rq = Pc * ir2;
     if constexpr (NEAR) {
-      if (!(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
+      if (!(r2 > hc.near2)) hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rq);
     }
     const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
-    const float dn = fmaf(dx, nx, fmaf(dy, ny, dz * nz));
+    const float dn5 = fmaf(dx, nx5, fmaf(dy, ny5, dz * nz5));
     const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
-    const float e = fmaf(Rc, dm * dn, Pc * mn);
-    const float f = Pc * dn;
+    const float e = fmaf(Rq, dm * dn5, Pc * mn);
+    const float f5 = Pc * dn5;
     dx0 = fmaf(dx, e, dx0);
     dy0 = fmaf(dy, e, dy0);
     dz0 = fmaf(dz, e, dz0);
-    mx0 = fmaf(mx, f, mx0);
-    my0 = fmaf(my, f, my0);
-    mz0 = fmaf(mz, f, mz0);
+    mx0 = fmaf(mx, f5, mx0);
+    my0 = fmaf(my, f5, my0);
+    mz0 = fmaf(mz, f5, mz0);
     n0 = fmaf(Pc, dm, n0);
   }
 }
@@ -1785,7 +1791,7 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
 __device__ __forceinline__ void near_force_full(const SrcRec *__restrict__ rec,
                                                 const int32_t *__restrict__ perm, int s0, int cnt,
                                                 const int2 *__restrict__ rg, int2 nrb,
-                                                const HessConsts &hc, const float *tab, int lane,
+                                                const HessConsts &hc, const float4 *tab, int lane,
                                                 const ForceOut &out) {
   int tb, te;
   p2p_target_span<64>(cnt, tb, te);
@@ -1794,16 +1800,17 @@ __device__ __forceinline__ void near_force_full(const SrcRec *__restrict__ rec,
     const int t = t0 + lane;
     const SrcRec me = rec[s0 + (t < te ? t : te - 1)];
     const float tx = me.x, ty = me.y, tz = me.z, nx = me.mx, ny = me.my, nz = me.mz;
+    const float nx5 = -5.f * nx, ny5 = -5.f * ny, nz5 = -5.f * nz;
     double ad0 = 0.0, ad1 = 0.0, ad2 = 0.0, am0 = 0.0, am1 = 0.0, am2 = 0.0, an = 0.0;
     for (int r = 0; r < nrange; ++r) {
       const int2 q = rg[r];
       float dx0 = 0.f, dy0 = 0.f, dz0 = 0.f, mx0 = 0.f, my0 = 0.f, mz0 = 0.f, n0 = 0.f;
       if (r < nrb.x)
-        force_range<1, true>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0, mx0,
-                             my0, mz0, n0);
+        force_range<1, true>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, nx5, ny5, nz5, hc, tab, dx0,
+                             dy0, dz0, mx0, my0, mz0, n0);
       else
-        force_range<1, false>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, hc, tab, dx0, dy0, dz0, mx0,
-                              my0, mz0, n0);
+        force_range<1, false>(rec, q.x, q.y, tx, ty, tz, nx, ny, nz, nx5, ny5, nz5, hc, tab, dx0,
+                              dy0, dz0, mx0, my0, mz0, n0);
       ad0 += dx0;
       ad1 += dy0;
       ad2 += dz0;
@@ -1812,9 +1819,11 @@ __device__ __forceinline__ void near_force_full(const SrcRec *__restrict__ rec,
       am2 += mz0;
       an += n0;
     }
+    // the m-sums hold -5 times their terms (force_range)
     if (t < te)
-      out.put(perm[s0 + t], ad0 + am0 + static_cast<double>(nx) * an,
-              ad1 + am1 + static_cast<double>(ny) * an, ad2 + am2 + static_cast<double>(nz) * an);
+      out.put(perm[s0 + t], ad0 - 0.2 * am0 + static_cast<double>(nx) * an,
+              ad1 - 0.2 * am1 + static_cast<double>(ny) * an,
+              ad2 - 0.2 * am2 + static_cast<double>(nz) * an);
   }
 }
 
@@ -1823,7 +1832,7 @@ __device__ __forceinline__ void near_force_remainder(const SrcRec *__restrict__ 
                                                      const int32_t *__restrict__ perm, int s0,
                                                      int cnt, const int *s_pre, const int *s_beg,
                                                      int nrange, int nmix, const HessConsts &hc,
-                                                     const float *tab, int lane,
+                                                     const float4 *tab, int lane,
                                                      const ForceOut &out) {
   int tb, te;
   remainder_span(cnt, tb, te);
@@ -1860,7 +1869,10 @@ __device__ __forceinline__ void near_force_remainder(const SrcRec *__restrict__ 
             const float ir2 = ir * ir;
             float Pc = ir2 * ir2 * ir;
             float Rc = -5.f * Pc * ir2;
-            if (mixed && !(r2 > hc.near2)) hess_near_coeffs(tab, hc.inv_h, r2, ir, Pc, Rc);
+            if (mixed && !(r2 > hc.near2)) {
+              hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rc);
+              Rc *= -5.f;
+            }
             const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
             const float dn = fmaf(dx, nx[t], fmaf(dy, ny[t], dz * nz[t]));
             const float mn = fmaf(mx, nx[t], fmaf(my, ny[t], mz * nz[t]));
@@ -1897,20 +1909,17 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
     const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const SrcRec *__restrict__ rec, const int32_t *__restrict__ perm,
     const int2 *__restrict__ rng, const int2 *__restrict__ nr, int nslot, HessConsts hc,
-    ForceOut out) {
+    const float4 *__restrict__ htab, ForceOut out) {
   static_assert(T == 64, "one wave per leaf");
   bool rem;
   const int li = xcd_two_lists(nrem, nfull, rem);
   if (li < 0) return;  // uniform over the workgroup, before the barrier
   const int b = rem ? remain[li] : full[li];
   const int lane = threadIdx.x;
-  __shared__ float4 s_tab[12];
+  __shared__ float4 s_tab[2 * kHessTabN];
   __shared__ int s_pre[kNearH + 1], s_beg[kNearH];
-  if (lane < 12) {
-    const float *row = &hc.curve[lane / 3][(lane % 3) * 4];
-    s_tab[lane] = make_float4(row[0], row[1], row[2], row[3]);
-  }
-  const float *tab = reinterpret_cast<const float *>(s_tab);
+  for (int i = lane; i < 2 * kHessTabN; i += 64) s_tab[i] = htab[i];
+  const float4 *tab = s_tab;
   const int2 nrb = nr[b];
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   if (rem) range_prefix(s_pre, s_beg, rg, nrb.x + nrb.y, lane);
@@ -2047,10 +2056,11 @@ struct FmmTables {
   int slot_ld = 0;
 };
 
-// Near-pair coefficient table of one smoothing length (make_field_table),
-// built once per context and h.
+// Near-pair coefficient tables of one smoothing length, built once per
+// context and h: the field's (make_field_table) and the force's
+// (make_hess_table).
 struct FieldTable {
-  DeviceBuffer buf;  // kFieldTabN x 8 floats
+  DeviceBuffer buf;  // kFieldTabN or kHessTabN x 8 floats
 };
 
 struct FmmState {
@@ -2063,7 +2073,7 @@ struct FmmState {
   int nmixH = 0;  // leading force near slots that may hold pairs within 4h
   int64_t n = 0;
   std::shared_ptr<FmmTables> tab;
-  std::shared_ptr<FieldTable> ftab;
+  std::shared_ptr<FieldTable> ftab, htab;
   DeviceBuffer keys, keys_sorted, idx, perm, flags, scan, rec;  // particle arrays
   // boxes: level l occupies [base[l], base[l] + count[l]) of the global arrays
   std::vector<int> base, count;
@@ -2274,19 +2284,20 @@ Status build_tables(mf_ctx *ctx, int P, std::shared_ptr<FmmTables> *out) {
   return Status::Ok();
 }
 
-Status build_field_table(mf_ctx *ctx, double h, std::shared_ptr<FieldTable> *out) {
+Status build_field_table(mf_ctx *ctx, double h, bool hess, std::shared_ptr<FieldTable> *out) {
   static std::mutex mu;
   std::lock_guard<std::mutex> lock(mu);
   char key[64];
-  std::snprintf(key, sizeof(key), "field_table_h%a", h);
+  std::snprintf(key, sizeof(key), hess ? "hess_table_h%a" : "field_table_h%a", h);
   auto it = ctx->cache.find(key);
   if (it != ctx->cache.end()) {
     *out = std::static_pointer_cast<FieldTable>(it->second);
     return Status::Ok();
   }
   auto t = std::make_shared<FieldTable>();
-  std::vector<float> tab(8 * kFieldTabN);
-  make_field_table(h, tab.data());
+  std::vector<float> tab(8 * (hess ? kHessTabN : kFieldTabN));
+  if (hess) make_hess_table(h, tab.data());
+  else make_field_table(h, tab.data());
   MF_TRY(t->buf.reserve(tab.size() * sizeof(float)));
   MF_HIP(hipMemcpy(t->buf.get(), tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
   ctx->cache[key] = t;
@@ -2321,7 +2332,8 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     return Status::Err(MF_ERR_INVALID_ARG, "FMM expansion order must be 6, 8 or 10");
   F.KP = kp_for(F.P);
   MF_TRY(build_tables(ctx, F.P, &F.tab));
-  MF_TRY(build_field_table(ctx, plan->h, &F.ftab));
+  MF_TRY(build_field_table(ctx, plan->h, false, &F.ftab));
+  MF_TRY(build_field_table(ctx, plan->h, true, &F.htab));
   const FmmTables &tab = *F.tab;
   const int64_t n = plan->n;
   F.n = n;
@@ -3146,7 +3158,7 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
                              lists + F.leaf_list_off[1], nrem, F.box_first.as<int32_t>(),
                              F.box_count.as<int32_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
                              F.rngH.as<int2>(), F.nrH.as<int2>(), kNearH, plan->hc,
-                             ForceOut{0, fx, fy, fz});
+                             F.htab->buf.as<float4>(), ForceOut{0, fx, fy, fz});
           MF_LAUNCH_CHECK();
         }
       }

@@ -63,15 +63,27 @@ struct HessConsts {
   float h;
   float inv_h;
   float near2;       // (4h)^2
+  float tab_scale;   // kHessTabN / (4h): r -> table coordinate q kHessTabN / 4
   // curve[k][0..4] = C1 re-centred at k+0.5 (ascending powers),
   // curve[k][5..9] = C2 - 3 C1 re-centred likewise, both times 1/(s h^4),
   // s = 3 mu0/(4 pi), in FP64 before rounding; [10..11] padding.
   float curve[4][12];
 };
 
+// Force-curve table of the FMM near field: kHessTabN intervals over q in
+// (0, 4], 16 per unit, each inside one of the reference's quartic pieces
+// (piece p covers (p, p + 1], so an integer q stays with the lower piece).
+// Interval k covers u = q kHessTabN / 4 in (k, k + 1] and holds two cubics in
+// t = u - (k + 1), in (-1, 0], each interpolating at the interval's four
+// Chebyshev points: [0..3] C1 / (s h^4), [4..7] -(C2 - 3 C1) / (5 s h^4), the
+// R / -5 the FMM force pair uses.  Within 9e-8 of the largest curve value
+// (the FP32 rounding of the curves is 6e-8).
+constexpr int kHessTabN = 64;
+
 FieldConsts make_field_consts(double h);
 HessConsts make_hess_consts(double h);
 void make_field_table(double h, float *tab);  // kFieldTabN x 8 floats
+void make_hess_table(double h, float *tab);   // kHessTabN x 8 floats
 
 // ---------------------------------------------------------------------------
 // Field pair coefficients (FP32).  Returns alpha, beta such that the pair adds
@@ -135,6 +147,24 @@ __device__ __forceinline__ void hess_near_coeffs(const float *tab, float inv_h, 
   const bool live = r2 > 0.f;
   P = live ? c1 * ir : 0.f;
   R = live ? dd * (ir * ir * ir) : 0.f;
+}
+
+// P and R / -5 of a force pair within 4h (r2 <= near2) from the table (LDS).
+// The inverse distance is clamped to a finite value, so a pair at d = 0 (a
+// particle and itself, or two coincident ones) gets finite coefficients,
+// whose terms all vanish with d.
+__device__ __forceinline__ void hess_near_coeffs_tab(const float4 *tab, float tab_scale, float r2,
+                                                     float ir, float &P, float &Rq) {
+  const float irc = fminf(ir, 1e12f);
+  const float u = r2 * irc * tab_scale;  // in [0, kHessTabN], up to rounding
+  const float kc = __builtin_amdgcn_fmed3f(ceilf(u), 1.f, static_cast<float>(kHessTabN));
+  const float t = u - kc;
+  const int k = static_cast<int>(kc) - 1;
+  const float4 a = tab[2 * k], b = tab[2 * k + 1];
+  const float c1 = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
+  const float dq = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
+  P = c1 * irc;
+  Rq = dq * (irc * irc * irc);
 }
 
 }  // namespace mf
