@@ -925,3 +925,13 @@ Status hessian_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int64_t nt, cons
 }
 
 }  // namespace mf
+
+// Internal: the near-pair coefficient tables, for the CPU tests (not part of
+// mf_capi.h).  kind 0 = field (kFieldTabN x 8 floats), 1 = force (kHessTabN x 8).
+extern "C" int mf_internal_near_table(int kind, double h, float *out) {
+  if (!out || !(h > 0.0)) return 1;
+  if (kind == 0) mf::make_field_table(h, out);
+  else if (kind == 1) mf::make_hess_table(h, out);
+  else return 1;
+  return 0;
+}
