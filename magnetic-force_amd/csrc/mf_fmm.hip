@@ -262,12 +262,62 @@ __global__ void run_counts_kernel(int nrun, int total, const int32_t *__restrict
   count[j] = (j + 1 < nrun ? first[j + 1] : total) - first[j];
 }
 
-// Parent index of every child box: parent = run index of the child.
-__global__ void set_parent_kernel(int nchild, const int32_t *__restrict__ child_run, int parent_base,
-                                  int32_t *__restrict__ parent) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nchild) return;
-  parent[j] = parent_base + child_run[j];
+// Boxes of every level above the leaves at once.  Level l < D holds the
+// distinct prefixes key >> 3 (D - l) of the sorted leaf keys and sits at
+// [base[l], base[l] + cnt[l]) of the box arrays, the levels stored after the
+// leaves from D - 1 down to 0.  Row r = D - 1 - l of the (level, leaf) flag
+// array marks the leaves that start a box of level l; rows are laid out in
+// that order, so an exclusive scan over all rows gives every flagged leaf its
+// box's global index minus cnt[D].
+struct LevelTable {
+  int D;
+  int base[kMaxLevels];
+  int cnt[kMaxLevels];
+};
+
+__global__ void level_flags_kernel(int nleaf, const uint64_t *__restrict__ keys, int D,
+                                   int32_t *__restrict__ flags) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<int64_t>(D) * nleaf) return;
+  const int r = static_cast<int>(e / nleaf), i = static_cast<int>(e % nleaf);
+  const int sh = 3 * (r + 1);  // level l = D - 1 - r: shift 3 (D - l)
+  flags[e] = (i == 0 || (keys[i] >> sh) != (keys[i - 1] >> sh)) ? 1 : 0;
+}
+
+// Per (level l, leaf i): a box starting at i gets its key and first child
+// (level-local index of leaf i's level l + 1 box); a level l + 1 box starting
+// at i gets its parent (leaf i's level l box).
+__global__ void level_boxes_kernel(int nleaf, const uint64_t *__restrict__ keys, LevelTable lt,
+                                   const int32_t *__restrict__ flags,
+                                   const int32_t *__restrict__ scan, uint64_t *__restrict__ box_keys,
+                                   int32_t *__restrict__ box_first, int32_t *__restrict__ box_parent) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= static_cast<int64_t>(lt.D) * nleaf) return;
+  const int r = static_cast<int>(e / nleaf), i = static_cast<int>(e % nleaf);
+  const int l = lt.D - 1 - r;
+  const int g = nleaf + scan[e] + flags[e] - 1;  // leaf i's level l box
+  // leaf i's level l + 1 box (at l + 1 = D, the leaf itself), and whether i starts it
+  const int64_t ec = e - nleaf;
+  const int gc = r == 0 ? i : nleaf + scan[ec] + flags[ec] - 1;
+  const bool child_start = r == 0 || flags[ec] != 0;
+  if (flags[e]) {
+    box_keys[g] = keys[i] >> (3 * (r + 1));
+    box_first[g] = gc - lt.base[l + 1];
+  }
+  if (child_start) box_parent[gc] = g;
+}
+
+// Child counts of the boxes above the leaves: first child of the next box of
+// the level (or the child level's box count) minus this box's first child.
+__global__ void level_child_counts_kernel(int nupper, LevelTable lt,
+                                          const int32_t *__restrict__ box_first,
+                                          int32_t *__restrict__ box_count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x + lt.cnt[lt.D];
+  if (j >= lt.cnt[lt.D] + nupper) return;
+  int l = lt.D - 1;
+  while (l > 0 && j >= lt.base[l] + lt.cnt[l]) --l;
+  const int end = j + 1 < lt.base[l] + lt.cnt[l] ? box_first[j + 1] : lt.cnt[l + 1];
+  box_count[j] = end - box_first[j];
 }
 
 // Near-field lists of the leaves: list[b * nslot + k] = leaf index or -1.
@@ -2460,15 +2510,12 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
   MF_TRY(need<int32_t>(F.box_first, cap));
   MF_TRY(need<int32_t>(F.box_count, cap));
   MF_TRY(need<int32_t>(F.box_parent, cap));
-  // place levels: leaves first at base 0 temporarily; final bases assigned below
-  // Layout: levels stored from leaf (D) at the end; compute counts level by level.
+  // Box arrays: the leaves (level D) at base 0, then levels D - 1 .. 0:
+  // base[D - 1] = cnt[D], base[D - 2] = cnt[D] + cnt[D - 1], ...
   std::vector<int> cnt(D + 1);
   cnt[D] = F.count[D];
-  // Leaf arrays are written at offset 0 of a scratch area, then levels above.
-  // We store level l at base[l]; choose base[D] = 0, base[D-1] = cnt[D], ...
-  int cursor = 0;
+  int cursor = cnt[D];
   F.base[D] = 0;
-  cursor = cnt[D];
   hipLaunchKernelGGL(run_starts_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n,
                      F.keys_sorted.as<uint64_t>(), 0, F.flags.as<int32_t>(), F.scan.as<int32_t>(),
                      F.box_keys.as<uint64_t>(), F.box_first.as<int32_t>(),
@@ -2489,31 +2536,34 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     MF_TRY(read_ints(ctx, static_cast<const int32_t *>(pc), D, lc));
     for (int l = 0; l < D; ++l) cnt[l] = lc[l] + 1;
   }
-  // level-local child run of each child box (the largest level first, so the
-  // workspace does not grow inside the loop)
-  void *pr = nullptr;
-  MF_TRY(workspace(ctx, "fmm_child_run", sizeof(int32_t) * (cnt[D] + 1), &pr));
-  for (int l = D - 1; l >= 0; --l) {
-    const int nc = cnt[l + 1];
-    const uint64_t *ck = F.box_keys.as<uint64_t>() + F.base[l + 1];
-    hipLaunchKernelGGL(run_flags_kernel, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream,
-                       static_cast<int64_t>(nc), ck, 3, F.flags.as<int32_t>());
+  // every level above the leaves in one pass (level_flags / scan /
+  // level_boxes / level_child_counts)
+  if (D > 0) {
+    LevelTable lt{};
+    lt.D = D;
+    for (int l = D; l >= 0; --l) {
+      lt.cnt[l] = cnt[l];
+      lt.base[l] = l == D ? 0 : cursor;
+      if (l < D) cursor += cnt[l];
+    }
+    for (int l = 0; l < D; ++l) F.base[l] = lt.base[l];
+    const int64_t ne = static_cast<int64_t>(D) * cnt[D];
+    MF_TRY(need<int32_t>(F.flags, ne));
+    MF_TRY(need<int32_t>(F.scan, ne));
+    const uint64_t *lk = F.box_keys.as<uint64_t>();
+    hipLaunchKernelGGL(level_flags_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, ctx->stream,
+                       cnt[D], lk, D, F.flags.as<int32_t>());
     MF_LAUNCH_CHECK();
-    MF_TRY(exclusive_scan_i32(ctx, F.flags.as<int32_t>(), F.scan.as<int32_t>(), nc));
-    F.base[l] = cursor;
-    // parent keys/first child (level-local child index), child -> parent
-    hipLaunchKernelGGL(run_starts_kernel, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream,
-                       static_cast<int64_t>(nc), ck, 3, F.flags.as<int32_t>(), F.scan.as<int32_t>(),
-                       F.box_keys.as<uint64_t>() + cursor, F.box_first.as<int32_t>() + cursor,
-                       static_cast<int32_t *>(pr));
+    MF_TRY(exclusive_scan_i32(ctx, F.flags.as<int32_t>(), F.scan.as<int32_t>(), ne));
+    hipLaunchKernelGGL(level_boxes_kernel, dim3(grid_for(ne, 256)), dim3(256), 0, ctx->stream,
+                       cnt[D], lk, lt, F.flags.as<int32_t>(), F.scan.as<int32_t>(),
+                       F.box_keys.as<uint64_t>(), F.box_first.as<int32_t>(),
+                       F.box_parent.as<int32_t>());
     MF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(run_counts_kernel, dim3(grid_for(cnt[l], 256)), dim3(256), 0, ctx->stream,
-                       cnt[l], nc, F.box_first.as<int32_t>() + cursor, F.box_count.as<int32_t>() + cursor);
+    const int nupper = cursor - cnt[D];
+    hipLaunchKernelGGL(level_child_counts_kernel, dim3(grid_for(nupper, 256)), dim3(256), 0,
+                       ctx->stream, nupper, lt, F.box_first.as<int32_t>(), F.box_count.as<int32_t>());
     MF_LAUNCH_CHECK();
-    hipLaunchKernelGGL(set_parent_kernel, dim3(grid_for(nc, 256)), dim3(256), 0, ctx->stream, nc,
-                       static_cast<const int32_t *>(pr), cursor, F.box_parent.as<int32_t>() + F.base[l + 1]);
-    MF_LAUNCH_CHECK();
-    cursor += cnt[l];
   }
   F.count = cnt;
   F.nbox = cursor;
