@@ -1307,16 +1307,30 @@ __global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_
   if (lane == 0) nr[b] = make_int2(n0, pos - n0);
 }
 
-// L2P, field: out[perm] = H_const + far field, or out[perm] += far field
-// (accumulate; the near-field kernels have written H_const + near).  One
-// wave per leaf.
+// out += far, per particle in caller order: the far part (L2P, and for the
+// force the external-tensor term) was written in FP64 on the context stream
+// while the near field ran beside it; the sum rounds once, as a single
+// kernel adding FP64 terms would.  Systems that have converged (batch plans)
+// are skipped, as in the other product kernels.
+__global__ void add_far_kernel(int64_t n, const double *__restrict__ far3, float *__restrict__ ox,
+                               float *__restrict__ oy, float *__restrict__ oz,
+                               const int32_t *__restrict__ sys_of, const int32_t *__restrict__ active) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (active != nullptr && active[sys_of[i]] == 0) return;
+  ox[i] = static_cast<float>(static_cast<double>(ox[i]) + far3[i]);
+  oy[i] = static_cast<float>(static_cast<double>(oy[i]) + far3[n + i]);
+  oz[i] = static_cast<float>(static_cast<double>(oz[i]) + far3[2 * n + i]);
+}
+
+// L2P, field: far3[.][perm] = far field (FP64; add_far_kernel adds it to
+// H_const + near).  One wave per leaf.
 template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_field_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
-    const int32_t *__restrict__ perm, const float *__restrict__ Lt, Frame fr, double s, float hc0,
-    float hc1, float hc2, int accumulate, float *__restrict__ ox, float *__restrict__ oy,
-    float *__restrict__ oz, SysMask mask) {
+    const int32_t *__restrict__ perm, const float *__restrict__ Lt, Frame fr, double s,
+    int64_t n, double *__restrict__ far3, SysMask mask) {
   const int b = blockIdx.x;
   if (sys_skip(mask, b)) return;
   const int lane = threadIdx.x;
@@ -1329,22 +1343,14 @@ __global__ __launch_bounds__(64) void l2p_field_kernel(
   for (int t0 = 0; t0 < cnt; t0 += 64) {
     const int t = t0 + lane;
     const SrcRec me = rec[s0 + (t < cnt ? t : cnt - 1)];
-    // the near-field values are read before the expansion is evaluated, so
-    // the scattered loads overlap the arithmetic
-    const int o = perm[s0 + (t < cnt ? t : cnt - 1)];
-    double b0 = hc0, b1 = hc1, b2 = hc2;
-    if (accumulate) {
-      b0 = static_cast<double>(ox[o]);
-      b1 = static_cast<double>(oy[o]);
-      b2 = static_cast<double>(oz[o]);
-    }
     float gx, gy, gz;
     l2p_grad<P>(L, static_cast<float>(me.x - cx) * inv_s, static_cast<float>(me.y - cy) * inv_s,
                 static_cast<float>(me.z - cz) * inv_s, gx, gy, gz);
     if (t < cnt) {
-      ox[o] = static_cast<float>(b0 + far_scale * gx);
-      oy[o] = static_cast<float>(b1 + far_scale * gy);
-      oz[o] = static_cast<float>(b2 + far_scale * gz);
+      const int o = perm[s0 + t];
+      far3[o] = far_scale * gx;
+      far3[n + o] = far_scale * gy;
+      far3[2 * n + o] = far_scale * gz;
     }
   }
 }
@@ -1545,16 +1551,14 @@ struct Text9 {
   const float *p[9];
 };
 
-// L2P, force: F[perm] = (far tensor + T_ext) . m, or F[perm] += that
-// (accumulate; the near-field kernels have written the near force).  One
-// wave per leaf.
+// L2P, force: far3[.][perm] = (far tensor + T_ext) . m (FP64;
+// add_far_kernel adds it to the near force).  One wave per leaf.
 template <int P, int KP>
 __global__ __launch_bounds__(64) void l2p_force_kernel(
     int nleaf, const int32_t *__restrict__ start, const int32_t *__restrict__ count,
     const uint64_t *__restrict__ keys, const SrcRec *__restrict__ rec,
     const int32_t *__restrict__ perm, const float *__restrict__ Lt, int use_far, Frame fr,
-    double s, Text9 text, int has_text, int accumulate, float *__restrict__ fx,
-    float *__restrict__ fy, float *__restrict__ fz) {
+    double s, Text9 text, int has_text, int64_t n, double *__restrict__ far3) {
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   double cx, cy, cz;
@@ -1590,14 +1594,9 @@ __global__ __launch_bounds__(64) void l2p_force_kernel(
         F1 += v[1];
         F2 += v[2];
       }
-      if (accumulate) {
-        F0 += static_cast<double>(fx[o]);
-        F1 += static_cast<double>(fy[o]);
-        F2 += static_cast<double>(fz[o]);
-      }
-      fx[o] = static_cast<float>(F0);
-      fy[o] = static_cast<float>(F1);
-      fz[o] = static_cast<float>(F2);
+      far3[o] = F0;
+      far3[n + o] = F1;
+      far3[2 * n + o] = F2;
     }
   }
 }
@@ -3132,10 +3131,17 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
   const int use_far = D >= 2 ? 1 : 0;
   SysMask mask{};
   if (plan->active_sys && F.nsys > 1) mask = SysMask{F.box_sys.as<int32_t>(), plan->active_sys};
+  double *far3 = nullptr;
+  if (use_far) {
+    void *pf = nullptr;
+    MF_TRY(workspace(ctx, "fmm_far3", sizeof(double) * 3 * static_cast<size_t>(n), &pf));
+    far3 = static_cast<double *>(pf);
+  }
   {
     // The near field (VALU) runs on the auxiliary stream, concurrently with
-    // the far-field chain (P2M .. L2L); it writes H_const + near, and the L2P
-    // adds the far field once both are done.
+    // the far-field chain (P2M .. L2L, L2P); it writes H_const + near, the
+    // L2P writes the far field into FP64 scratch, and add_far_kernel sums
+    // them once both are done.
     NearOverlap ov(ctx);
     const bool serial = plan->opts.serial_near != 0;
     auto near = [&]() -> Status {
@@ -3162,15 +3168,24 @@ Status apply_field_impl(mf_plan *plan, float scale, const float *px, const float
       return serial ? Status::Ok() : ov.record();
     };
     MF_TRY((upward_and_m2l<P, KP>(plan, false, near, mask)));
+    if (use_far) {
+      // the far part into FP64 scratch on the context stream, beside the
+      // near field; added once both are done
+      KernelTimer timer(ctx, "l2p_field");
+      hipLaunchKernelGGL((l2p_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                         F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
+                         F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                         F.Lt.as<float>(), Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, n, far3,
+                         mask);
+      MF_LAUNCH_CHECK();
+    }
     MF_TRY(ov.join());
   }
   if (use_far) {
-    KernelTimer timer(ctx, "l2p_field");
-    hipLaunchKernelGGL((l2p_field_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                       F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
-                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
-                       F.Lt.as<float>(), Frame{F.sys_lo.as<double>(), 3 * D}, F.s_leaf, h0, h1, h2,
-                       1, ox, oy, oz, mask);
+    hipLaunchKernelGGL(add_far_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, far3,
+                       ox, oy, oz,
+                       mask.active ? plan->sys_of.as<int32_t>() : static_cast<const int32_t *>(nullptr),
+                       mask.active);
     MF_LAUNCH_CHECK();
   }
   return Status::Ok();
@@ -3187,9 +3202,25 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
                      n, F.perm.as<int32_t>(), mx, my, mz, 1.f, static_cast<const float *>(nullptr),
                      F.rec.as<SrcRec>());
   MF_LAUNCH_CHECK();
+  Text9 t{};
+  int has_text = 0;
+  if (text9) {
+    has_text = 1;
+    for (int k = 0; k < 9; ++k) {
+      MF_ARG(text9[k] != nullptr, "text9 entries must be non-null device pointers");
+      t.p[k] = text9[k];
+    }
+  }
+  double *far3 = nullptr;
+  {
+    void *pf = nullptr;
+    MF_TRY(workspace(ctx, "fmm_far3", sizeof(double) * 3 * static_cast<size_t>(n), &pf));
+    far3 = static_cast<double *>(pf);
+  }
   {
     // near force on the auxiliary stream (writes), concurrently with the
-    // far-field chain; the L2P adds the far part and T_ext . m
+    // far-field chain; the L2P writes the far part and T_ext . m into FP64
+    // scratch, and add_far_kernel sums them once both are done
     NearOverlap ov(ctx);
     const bool serial = plan->opts.serial_near != 0;
     auto near = [&]() -> Status {
@@ -3216,27 +3247,23 @@ Status apply_force_impl(mf_plan *plan, const float *mx, const float *my, const f
       return serial ? Status::Ok() : ov.record();
     };
     MF_TRY((upward_and_m2l<P, KP>(plan, true, near, SysMask{})));
+    {
+      // far part + T_ext . m into FP64 scratch, beside the near field
+      KernelTimer timer(ctx, "l2p_force");
+      const int D = F.D, nleaf = F.count[D];
+      hipLaunchKernelGGL((l2p_force_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+                         F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
+                         F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
+                         F.Lt.as<float>(), D >= 2 ? 1 : 0, Frame{F.sys_lo.as<double>(), 3 * D},
+                         F.s_leaf, t, has_text, n, far3);
+      MF_LAUNCH_CHECK();
+    }
     MF_TRY(ov.join());
   }
-  Text9 t{};
-  int has_text = 0;
-  if (text9) {
-    has_text = 1;
-    for (int k = 0; k < 9; ++k) {
-      MF_ARG(text9[k] != nullptr, "text9 entries must be non-null device pointers");
-      t.p[k] = text9[k];
-    }
-  }
-  const int D = F.D, nleaf = F.count[D];
-  {
-    KernelTimer timer(ctx, "l2p_force");
-    hipLaunchKernelGGL((l2p_force_kernel<P, KP>), dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
-                       F.box_first.as<int32_t>(), F.box_count.as<int32_t>(),
-                       F.box_keys.as<uint64_t>(), F.rec.as<SrcRec>(), F.perm.as<int32_t>(),
-                       F.Lt.as<float>(), D >= 2 ? 1 : 0, Frame{F.sys_lo.as<double>(), 3 * D},
-                       F.s_leaf, t, has_text, 1, fx, fy, fz);
-    MF_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(add_far_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, n, far3,
+                     fx, fy, fz, static_cast<const int32_t *>(nullptr),
+                     static_cast<const int32_t *>(nullptr));
+  MF_LAUNCH_CHECK();
   return Status::Ok();
 }
 
