@@ -124,6 +124,18 @@ def test_pool8m_shape_bf16x3_force_vs_fp64_solution(ctx):
     print(f"pool8m-shape: x_err={ex:.3e} F_err={ef:.3e} iters={res.stats['iterations']}")
     assert ex < 5e-6
     assert ef < 1e-5
+    # the bench's setting: products made below a residual of 1e-2 on the
+    # degree-cut one-part translations
+    rel = plan.solve(kappa, b, rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1_CUT,
+                     relax_rtol=1e-2)
+    assert rel.stats["converged"] == 1 and rel.stats["relaxed_matvecs"] >= 1
+    Fr = plan.force(kappa * rel.x)
+    efr = tnorm_rel(Fr.double(), F_ref)
+    exr = tnorm_rel(rel.x.double(), ref.x)
+    print(f"pool8m-shape relaxed: x_err={exr:.3e} F_err={efr:.3e} "
+          f"relaxed {rel.stats['relaxed_matvecs']} of {rel.stats['matvecs']}")
+    assert exr < 5e-6
+    assert efr < 1e-5
     plan.close()
     ap.close()
 
