@@ -1245,9 +1245,10 @@ __device__ __forceinline__ void l2p_hess(const float *__restrict__ Lt, float x, 
 // [0, nmix) class 0 and [nmix, nslot) class 1, are bitonic-sorted in LDS;
 // runs of consecutive keys are the ranges, emitted in key order (class 0
 // first), so the summation order is fixed.  rng[leaf * nslot + r] =
-// {first, end}; nr[leaf] = {class-0 ranges, class-1 ranges}.
-constexpr int kSortN = 256;  // >= kNearH
-
+// {first, end}; nr[leaf] = {class-0 ranges, class-1 ranges}.  kSortN, the
+// sort size, is a power of two >= nslot (128 for the field's 81 slots, 256
+// for the force's 179).
+template <int kSortN>
 __global__ __launch_bounds__(64) void near_ranges_kernel(int nleaf, const int32_t *__restrict__ list,
                                                          int nslot, int nmix,
                                                          const int32_t *__restrict__ first,
@@ -2123,6 +2124,8 @@ struct FmmState {
   int64_t n = 0;
   std::shared_ptr<FmmTables> tab;
   std::shared_ptr<FieldTable> ftab, htab;
+  // host copies of the tile descriptors (sources of asynchronous uploads)
+  std::vector<M2LTile> h_tilesF, h_tilesH, h_tilesM, h_tilesL;
   DeviceBuffer keys, keys_sorted, idx, perm, flags, scan, rec;  // particle arrays
   // boxes: level l occupies [base[l], base[l] + count[l]) of the global arrays
   std::vector<int> base, count;
@@ -2601,11 +2604,12 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
   MF_TRY(need<int2>(F.rngH, static_cast<int64_t>(nleaf) * kNearH));
   MF_TRY(need<int2>(F.nrF, nleaf));
   MF_TRY(need<int2>(F.nrH, nleaf));
-  hipLaunchKernelGGL(near_ranges_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+  static_assert(kNearF <= 128 && kNearH <= 256, "near-range sort sizes");
+  hipLaunchKernelGGL(near_ranges_kernel<128>, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                      F.nearF.as<int32_t>(), kNearF, kNearFMixed, F.box_first.as<int32_t>(),
                      F.box_count.as<int32_t>(), F.rngF.as<int2>(), F.nrF.as<int2>());
   MF_LAUNCH_CHECK();
-  hipLaunchKernelGGL(near_ranges_kernel, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
+  hipLaunchKernelGGL(near_ranges_kernel<256>, dim3(nleaf), dim3(64), 0, ctx->stream, nleaf,
                      F.nearH.as<int32_t>(), kNearH, F.nmixH, F.box_first.as<int32_t>(),
                      F.box_count.as<int32_t>(), F.rngH.as<int2>(), F.nrH.as<int2>());
   MF_LAUNCH_CHECK();
@@ -2684,7 +2688,11 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
                               F.class_ids.as<int32_t>(), F.class_ids2.as<int32_t>(), nm2l, 0, 8));
     std::vector<int32_t> cc;
     MF_TRY(read_ints(ctx, F.class_counts.as<int32_t>(), 8 * (kMaxLevels + 1), cc));
-    std::vector<M2LTile> tf, th;
+    // the host tile descriptors live in the plan until its next build, so the
+    // asynchronous copies below need no wait
+    std::vector<M2LTile> &tf = F.h_tilesF, &th = F.h_tilesH;
+    tf.clear();
+    th.clear();
     int pos = 0;
     for (int l = 2; l <= D; ++l) {
       if (l == D) F.leaf_tile_start = static_cast<int>(th.size());
@@ -2709,7 +2717,6 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     MF_TRY(need<M2LTile>(F.tilesH, F.ntilesH));
     MF_HIP(hipMemcpyAsync(F.tilesF.get(), tf.data(), sizeof(M2LTile) * tf.size(), hipMemcpyHostToDevice, ctx->stream));
     MF_HIP(hipMemcpyAsync(F.tilesH.get(), th.data(), sizeof(M2LTile) * th.size(), hipMemcpyHostToDevice, ctx->stream));
-    MF_HIP(hipStreamSynchronize(ctx->stream));  // host vectors go out of scope
     // non-empty offsets of every tile
     const int ld = tab.slot_ld;
     if (ld > 1024) return Status::Err(MF_ERR_UNSUPPORTED, "offset list longer than 1024");
@@ -2753,7 +2760,9 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
                          F.box_keys.as<uint64_t>() + F.base[l], F.child_tab.as<int32_t>());
       MF_LAUNCH_CHECK();
     }
-    std::vector<M2LTile> tm, tlv;
+    std::vector<M2LTile> &tm = F.h_tilesM, &tlv = F.h_tilesL;
+    tm.clear();
+    tlv.clear();
     F.tileM_start.assign(D + 2, 0);
     F.tileL_start.assign(D + 2, 0);
     for (int l = 2; l <= D; ++l) {  // M2M targets: parents at levels 2 .. D - 1
@@ -2810,7 +2819,6 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
                          F.tlistL.as<int32_t>(), F.tnL.as<int32_t>());
       MF_LAUNCH_CHECK();
     }
-    MF_HIP(hipStreamSynchronize(ctx->stream));  // host tile vectors go out of scope
   }
   if (nsys > 1) {
     MF_TRY(need<int32_t>(F.box_sys, F.nbox));
