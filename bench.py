@@ -98,7 +98,7 @@ def m2l_issued_per_tile_slot(order, m2l):
     offset).  FP32 and BF16x6: the whole KP x KP operator, 1 or 6 products.
     BF16x3 (degree-split kernel): per 16 x 32 block of the operator, 3
     products where the lowest local and multipole degrees in the block add up
-    to at most the order, 1 product elsewhere, none on pure padding."""
+    to at most the order minus 3, 1 product elsewhere, none on pure padding."""
     nd = (order + 1) ** 2
     kp = (nd + 31) // 32 * 32
     if m2l != "bf16x3":
@@ -108,7 +108,7 @@ def m2l_issued_per_tile_slot(order, m2l):
         for ks in range(kp // 32):
             if 16 * rb >= nd or 32 * ks >= nd:
                 continue
-            prods += 3 if math.isqrt(16 * rb) + math.isqrt(32 * ks) <= order else 1
+            prods += 3 if math.isqrt(16 * rb) + math.isqrt(32 * ks) <= order - 3 else 1
     return float(prods) * 4 * (16 * 16 * 32 * 2)  # 4 column blocks of 16 targets
 
 

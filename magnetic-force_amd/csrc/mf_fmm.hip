@@ -818,17 +818,20 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
 // (rho = box radius over separation, below 1/2 for the interaction lists
 // here).  These kernels sort the 16 x 32 MFMA blocks (row block rb: local
 // dofs 16 rb .. 16 rb + 15; k step ks: multipole dofs 32 ks .. 32 ks + 31)
-// by the lowest degrees in them: a block is "low" when those add up to at
-// most P.  At P = 10, 13 of the 32 blocks are low; every term of the other
-// 19 has j + n > P, and together they are ~4e-5 of a product on the 1.15 M
-// pool (measured against FP64).
-//  * MF_M2L_BF16X3 (SPLIT): the low blocks take the three-product split of
-//    m2l_kernel, the others one product of the leading BF16 parts (their
-//    ~4e-3 rounding of terms that small adds ~2e-7): the split's accuracy
-//    from 58 instead of 96 MFMAs per offset and column block, and 45
-//    instead of 64 KB of operator fragments.
-//  * MF_M2L_BF16X1_CUT (!SPLIT): the low blocks in one product, the others
-//    dropped: ~1e-4 relative per product, for the late products of a
+// by s, the sum of the lowest degrees in them.  At P = 10, 13 of the 32
+// blocks have s <= P; every term of the other 19 has j + n > P, and together
+// they are ~4e-5 of a product on the 1.15 M pool (measured against FP64).
+//  * MF_M2L_BF16X3 (SPLIT): the blocks with s <= P - 3 (5 of 32 at P = 10)
+//    take the three-product split of m2l_kernel, the others one product of
+//    the leading BF16 parts.  The one-part rounding (~4e-3) of terms that
+//    small stays below the order-P truncation: on the pool the field error
+//    went from 3.466e-6 to 3.494e-6 and the force error from 3.195e-6 to
+//    3.219e-6 against three products on every s <= P block; at s <= P - 5
+//    the force error grew 1.8 %, at s <= P - 6 21 %.  42 instead of 96
+//    MFMAs per offset and column block, 37 instead of 64 KB of operator
+//    fragments.
+//  * MF_M2L_BF16X1_CUT (!SPLIT): the blocks with s <= P in one product, the
+//    others dropped: ~1e-4 relative per product, for the late products of a
 //    relaxed solve, whose errors may grow as the residual falls.
 // Dofs of degree n are [n^2, (n + 1)^2); padding dofs (>= (P + 1)^2) are zero
 // in every table and expansion.
@@ -844,20 +847,23 @@ struct DegreeBlocks {
   static constexpr int ND = (P + 1) * (P + 1);
   // blocks that hold real dofs (the others are zero padding, skipped)
   static constexpr bool real(int rb, int ks) { return 16 * rb < ND && 32 * ks < ND; }
-  static constexpr bool low(int rb, int ks) {
-    return real(rb, ks) && dof_degree(16 * rb) + dof_degree(32 * ks) <= P;
-  }
+  static constexpr int min_sum(int rb, int ks) { return dof_degree(16 * rb) + dof_degree(32 * ks); }
+  // kept by the cut: lowest degrees add up to at most P
+  static constexpr bool low(int rb, int ks) { return real(rb, ks) && min_sum(rb, ks) <= P; }
+  // three products in the split: lowest degrees add up to at most P - 3
+  static constexpr bool low3(int rb, int ks) { return real(rb, ks) && min_sum(rb, ks) <= P - 3; }
   // MFMA products of row block rb per column block
   static constexpr int cost(int rb, bool split) {
     int c = 0;
-    for (int k = 0; k < NK; ++k) c += low(rb, k) ? (split ? 3 : 1) : (split && real(rb, k) ? 1 : 0);
+    for (int k = 0; k < NK; ++k)
+      c += split ? (low3(rb, k) ? 3 : real(rb, k) ? 1 : 0) : (low(rb, k) ? 1 : 0);
     return c;
   }
 };
 
 // One wave's share: its row blocks, the blocks it multiplies (k-step order)
 // and the operator fragments it loads.  Row blocks go to waves longest
-// first, each to the least loaded wave (P = 10, KP = 128: 18, 14, 14, 12
+// first, each to the least loaded wave (P = 10, KP = 128: 12, 10, 10, 10
 // products per column block with SPLIT, against 24 per wave for m2l_kernel;
 // 4, 3, 3, 3 for the cut, against 8).
 struct DegWave {
@@ -892,7 +898,7 @@ constexpr DegWave deg_wave(int w) {
     if (owner[rb] == w) s.rb[s.nr++] = rb;
   for (int ks = 0; ks < C::NK; ++ks)
     for (int i = 0; i < s.nr; ++i) {
-      const bool lo = C::low(s.rb[i], ks);
+      const bool lo = SPLIT ? C::low3(s.rb[i], ks) : C::low(s.rb[i], ks);
       if (!lo && !(SPLIT && C::real(s.rb[i], ks))) continue;
       const int b = s.nb++;
       s.b_row[b] = i;
@@ -1028,8 +1034,8 @@ __device__ __forceinline__ void m2l_deg_wave(
   }
 }
 
-// Two waves per SIMD for the split (two operator sets of up to 13
-// fragments), three for the cut.
+// Two waves per SIMD for the split (two operator sets of up to 10
+// fragments at P = 10), three for the cut.
 template <int KP, int P, bool SPLIT>
 __global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(SPLIT ? 2 : 3)))
 void m2l_deg_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *__restrict__ Lt,
