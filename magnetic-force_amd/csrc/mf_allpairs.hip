@@ -634,24 +634,41 @@ void cubic_through(const double (&t)[4], const double (&y)[4], double (&c)[4]) {
 }
 }  // namespace
 
+// Coefficients (ascending powers) of c(t + s) from those of the cubic c(t).
+static void shift_cubic(const double *c, double s, double *out) {
+  out[0] = ((c[3] * s + c[2]) * s + c[1]) * s + c[0];
+  out[1] = (3.0 * c[3] * s + 2.0 * c[2]) * s + c[1];
+  out[2] = 3.0 * c[3] * s + c[2];
+  out[3] = c[3];
+}
+
+static void store_entry(float *tab, int e, const double *ca, const double *cb) {
+  for (int j = 0; j < 4; ++j) {
+    tab[8 * e + j] = static_cast<float>(ca[j]);
+    tab[8 * e + 4 + j] = static_cast<float>(cb[j]);
+  }
+}
+
 void make_field_table(double h, float *tab) {
   const double ih3 = 1.0 / (h * h * h), ih5 = ih3 / (h * h);
   double t[4];
-  for (int i = 0; i < 4; ++i) t[i] = 0.5 * std::cos((2 * i + 1) * kPi / 8.0);
+  for (int i = 0; i < 4; ++i) t[i] = 0.5 + 0.5 * std::cos((2 * i + 1) * kPi / 8.0);
+  double ca[4], cb[4];
   for (int k = 0; k < kFieldTabN; ++k) {
-    double ya[4], yb[4], ca[4], cb[4];
+    double ya[4], yb[4];
     for (int i = 0; i < 4; ++i) {
-      near_coeffs_f64((k + 0.5 + t[i]) * 2.0 / kFieldTabN, ya[i], yb[i]);
+      near_coeffs_f64((k + t[i]) * 2.0 / kFieldTabN, ya[i], yb[i]);
       ya[i] *= ih5;
       yb[i] *= ih3;
     }
     cubic_through(t, ya, ca);
     cubic_through(t, yb, cb);
-    for (int j = 0; j < 4; ++j) {
-      tab[8 * k + j] = static_cast<float>(ca[j]);
-      tab[8 * k + 4 + j] = static_cast<float>(cb[j]);
-    }
+    store_entry(tab, k, ca, cb);
   }
+  double sa[4], sb[4];  // the last interval's cubics, one interval on
+  shift_cubic(ca, 1.0, sa);
+  shift_cubic(cb, 1.0, sb);
+  store_entry(tab, kFieldTabN, sa, sb);
 }
 
 // Fitted curve table of the reference (Dipole_hessian.kern:53-61), highest
@@ -694,11 +711,12 @@ void make_hess_table(double h, float *tab) {
   constexpr int per_unit = kHessTabN / 4;
   double t[4];
   for (int i = 0; i < 4; ++i) t[i] = -0.5 + 0.5 * std::cos((2 * i + 1) * kPi / 8.0);
-  for (int k = 0; k < kHessTabN; ++k) {
-    const int piece = k / per_unit;
+  double first_c[4], first_d[4];
+  for (int e = 1; e <= kHessTabN; ++e) {  // entry e: u in (e - 1, e]
+    const int piece = (e - 1) / per_unit;
     double yc[4], yd[4], cc[4], cd[4];
     for (int i = 0; i < 4; ++i) {
-      const double q = (k + 1 + t[i]) / per_unit;
+      const double q = (e + t[i]) / per_unit;
       double c1 = 0.0, c2 = 0.0;
       for (int j = 0; j < 5; ++j) {
         c1 = c1 * q + kC1[piece][j];
@@ -709,11 +727,22 @@ void make_hess_table(double h, float *tab) {
     }
     cubic_through(t, yc, cc);
     cubic_through(t, yd, cd);
-    for (int j = 0; j < 4; ++j) {
-      tab[8 * k + j] = static_cast<float>(cc[j]);
-      tab[8 * k + 4 + j] = static_cast<float>(cd[j]);
+    store_entry(tab, e, cc, cd);
+    if (e == 1) {
+      std::copy(cc, cc + 4, first_c);
+      std::copy(cd, cd + 4, first_d);
+    }
+    if (e == kHessTabN) {  // entry kHessTabN + 1 continues this one
+      double sc[4], sd[4];
+      shift_cubic(cc, 1.0, sc);
+      shift_cubic(cd, 1.0, sd);
+      store_entry(tab, kHessTabN + 1, sc, sd);
     }
   }
+  double sc[4], sd[4];  // entry 0 (u = 0) continues entry 1
+  shift_cubic(first_c, -1.0, sc);
+  shift_cubic(first_d, -1.0, sd);
+  store_entry(tab, 0, sc, sd);
 }
 
 const double (*reference_c1_table())[5] { return kC1; }
@@ -927,7 +956,8 @@ Status hessian_allpairs_launch(mf_ctx *ctx, const float4 *tpos, int64_t nt, cons
 }  // namespace mf
 
 // Internal: the near-pair coefficient tables, for the CPU tests (not part of
-// mf_capi.h).  kind 0 = field (kFieldTabN x 8 floats), 1 = force (kHessTabN x 8).
+// mf_capi.h).  kind 0 = field (kFieldTabEntries x 8 floats), 1 = force
+// (kHessTabEntries x 8).
 extern "C" int mf_internal_near_table(int kind, double h, float *out) {
   if (!out || !(h > 0.0)) return 1;
   if (kind == 0) mf::make_field_table(h, out);

@@ -1830,9 +1830,9 @@ __global__ __launch_bounds__(64) void p2p_field_kernel(
   const int lane = threadIdx.x;
   // near-pair coefficients from the table (2 KB of LDS); the remainder's
   // stream offsets
-  __shared__ float4 s_ftab[2 * kFieldTabN];
+  __shared__ float4 s_ftab[2 * kFieldTabEntries];
   __shared__ int s_pre[kNearF + 1], s_beg[kNearF];
-  for (int i = lane; i < 2 * kFieldTabN; i += 64) s_ftab[i] = ftab[i];
+  for (int i = lane; i < 2 * kFieldTabEntries; i += 64) s_ftab[i] = ftab[i];
   const int2 nrb = nr[b];
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
   if (rem) range_prefix(s_pre, s_beg, rg, nrb.x + nrb.y, lane);
@@ -1972,9 +1972,9 @@ __global__ __launch_bounds__(64) void p2p_force_kernel(
   if (li < 0) return;  // uniform over the workgroup, before the barrier
   const int b = rem ? remain[li] : full[li];
   const int lane = threadIdx.x;
-  __shared__ float4 s_tab[2 * kHessTabN];
+  __shared__ float4 s_tab[2 * kHessTabEntries];
   __shared__ int s_pre[kNearH + 1], s_beg[kNearH];
-  for (int i = lane; i < 2 * kHessTabN; i += 64) s_tab[i] = htab[i];
+  for (int i = lane; i < 2 * kHessTabEntries; i += 64) s_tab[i] = htab[i];
   const float4 *tab = s_tab;
   const int2 nrb = nr[b];
   const int2 *__restrict__ rg = rng + static_cast<int64_t>(b) * nslot;
@@ -2116,7 +2116,7 @@ struct FmmTables {
 // context and h: the field's (make_field_table) and the force's
 // (make_hess_table).
 struct FieldTable {
-  DeviceBuffer buf;  // kFieldTabN or kHessTabN x 8 floats
+  DeviceBuffer buf;  // kFieldTabEntries or kHessTabEntries x 8 floats
 };
 
 struct FmmState {
@@ -2353,7 +2353,7 @@ Status build_field_table(mf_ctx *ctx, double h, bool hess, std::shared_ptr<Field
     return Status::Ok();
   }
   auto t = std::make_shared<FieldTable>();
-  std::vector<float> tab(8 * (hess ? kHessTabN : kFieldTabN));
+  std::vector<float> tab(8 * (hess ? kHessTabEntries : kFieldTabEntries));
   if (hess) make_hess_table(h, tab.data());
   else make_field_table(h, tab.data());
   MF_TRY(t->buf.reserve(tab.size() * sizeof(float)));

@@ -48,15 +48,18 @@ struct FieldConsts {
 };
 
 // Near-field coefficient table of the FMM near-field kernels.  Interval k of
-// kFieldTabN covers q in [2k, 2k + 2) / kFieldTabN (q = 1 is a boundary, so no
-// interval straddles the two forms) and holds two cubics in
-// t = q kFieldTabN / 2 - (k + 0.5), for alpha and for beta, each interpolating
-// the exact FP64 form at the four Chebyshev points of the interval:
-// tab[8k + 0..3] = alpha coefficients, tab[8k + 4..7] = beta coefficients
-// (ascending powers, h included).  At FP32 inputs q in [0, 2) the error is at
-// most 1.5e-7 of max(|alpha|, (2h)^-5) and of max(|beta|, (2h)^-3); the FP32
-// closed form of field_near_coeffs stays within 7e-7 and 1.6e-6.
+// kFieldTabN covers u = q kFieldTabN / 2 in [k, k + 1) (q = 1 is a boundary,
+// so no interval straddles the two forms) and holds two cubics in t = u - k,
+// for alpha and for beta, each interpolating the exact FP64 form at the four
+// Chebyshev points of the interval: tab[8k + 0..3] = alpha coefficients,
+// tab[8k + 4..7] = beta coefficients (ascending powers, h included).  Entry
+// kFieldTabN continues the last interval's cubics one interval further, so a
+// u that rounds up to kFieldTabN at q -> 2 reads a finite value with no
+// clamp.  At FP32 inputs q in [0, 2) the error is at most 1.5e-7 of
+// max(|alpha|, (2h)^-5) and of max(|beta|, (2h)^-3); the FP32 closed form of
+// field_near_coeffs stays within 7e-7 and 1.6e-6.
 constexpr int kFieldTabN = 64;
+constexpr int kFieldTabEntries = kFieldTabN + 1;
 
 // Force-tensor constants; the curve table goes to LDS for per-lane lookup.
 struct HessConsts {
@@ -73,17 +76,21 @@ struct HessConsts {
 // Force-curve table of the FMM near field: kHessTabN intervals over q in
 // (0, 4], 16 per unit, each inside one of the reference's quartic pieces
 // (piece p covers (p, p + 1], so an integer q stays with the lower piece).
-// Interval k covers u = q kHessTabN / 4 in (k, k + 1] and holds two cubics in
-// t = u - (k + 1), in (-1, 0], each interpolating at the interval's four
-// Chebyshev points: [0..3] C1 / (s h^4), [4..7] -(C2 - 3 C1) / (5 s h^4), the
-// R / -5 the FMM force pair uses.  Within 9e-8 of the largest curve value
-// (the FP32 rounding of the curves is 6e-8).
+// Entry e = ceil(u), u = q kHessTabN / 4, covers u in (e - 1, e] and holds two
+// cubics in t = u - e, in (-1, 0]: [0..3] C1 / (s h^4), [4..7]
+// -(C2 - 3 C1) / (5 s h^4), the R / -5 the FMM force pair uses.  Entries
+// 1 .. kHessTabN interpolate at their interval's four Chebyshev points;
+// entry 0 (u = 0, a pair at d = 0) continues entry 1's cubics and entry
+// kHessTabN + 1 (u rounded above kHessTabN at q -> 4) entry kHessTabN's, so
+// the lookup needs no clamp.  Within 9e-8 of the largest curve value (the
+// FP32 rounding of the curves is 6e-8).
 constexpr int kHessTabN = 64;
+constexpr int kHessTabEntries = kHessTabN + 2;
 
 FieldConsts make_field_consts(double h);
 HessConsts make_hess_consts(double h);
-void make_field_table(double h, float *tab);  // kFieldTabN x 8 floats
-void make_hess_table(double h, float *tab);   // kHessTabN x 8 floats
+void make_field_table(double h, float *tab);  // kFieldTabEntries x 8 floats
+void make_hess_table(double h, float *tab);   // kHessTabEntries x 8 floats
 
 // ---------------------------------------------------------------------------
 // Field pair coefficients (FP32).  Returns alpha, beta such that the pair adds
@@ -120,11 +127,13 @@ __device__ __forceinline__ void field_near_coeffs(const FieldConsts &fc, float r
 // r2 < (2h)^2: 12 VALU operations and two LDS reads instead of about 30.
 __device__ __forceinline__ void field_near_coeffs_tab(const float4 *tab, float tab_scale, float r2,
                                                       float ir, float &alpha, float &beta) {
-  float u = r2 * ir * tab_scale;  // q kFieldTabN / 2; NaN when r2 == 0
-  u = r2 > 0.f ? u : 0.f;
-  // u < kFieldTabN up to rounding at q -> 2: the last interval then takes it
-  const int k = min(static_cast<int>(u), kFieldTabN - 1);
-  const float t = u - (static_cast<float>(k) + 0.5f);
+  // u = q kFieldTabN / 2.  The clamp keeps the pair at d = 0 (ir = inf) at
+  // u = 0 instead of 0 x inf; u < kFieldTabN up to rounding at q -> 2, and
+  // entry kFieldTabN takes a u rounded up to it.
+  const float irc = __builtin_amdgcn_fmed3f(ir, 0.f, 1e30f);
+  const float u = r2 * tab_scale * irc;
+  const int k = static_cast<int>(u);
+  const float t = __builtin_amdgcn_fractf(u);
   const float4 a = tab[2 * k], b = tab[2 * k + 1];
   alpha = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
   beta = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
@@ -155,12 +164,12 @@ __device__ __forceinline__ void hess_near_coeffs(const float *tab, float inv_h, 
 // whose terms all vanish with d.
 __device__ __forceinline__ void hess_near_coeffs_tab(const float4 *tab, float tab_scale, float r2,
                                                      float ir, float &P, float &Rq) {
-  const float irc = fminf(ir, 1e12f);
+  const float irc = __builtin_amdgcn_fmed3f(ir, 0.f, 1e12f);
   const float u = r2 * irc * tab_scale;  // in [0, kHessTabN], up to rounding
-  const float kc = __builtin_amdgcn_fmed3f(ceilf(u), 1.f, static_cast<float>(kHessTabN));
+  const float kc = ceilf(u);              // entry 0 .. kHessTabN + 1
   const float t = u - kc;
-  const int k = static_cast<int>(kc) - 1;
-  const float4 a = tab[2 * k], b = tab[2 * k + 1];
+  const int e = static_cast<int>(kc);
+  const float4 a = tab[2 * e], b = tab[2 * e + 1];
   const float c1 = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
   const float dq = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
   P = c1 * irc;
