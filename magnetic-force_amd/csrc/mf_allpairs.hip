@@ -745,6 +745,14 @@ void make_hess_table(double h, float *tab) {
   store_entry(tab, 0, sc, sd);
 }
 
+void split_near_table(const float *tab, int entries, float *out) {
+  for (int e = 0; e < entries; ++e)
+    for (int j = 0; j < 4; ++j) {
+      out[4 * e + j] = tab[8 * e + j];
+      out[4 * (entries + e) + j] = tab[8 * e + 4 + j];
+    }
+}
+
 const double (*reference_c1_table())[5] { return kC1; }
 const double (*reference_c2_table())[5] { return kC2; }
 

@@ -1632,14 +1632,17 @@ __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int 
     const float ir = __builtin_amdgcn_rsqf(r2);
     const float ir2 = ir * ir;
     float Pc = ir2 * ir2 * ir;
-    float Rq = Pc * ir2;
-    if constexpr (NEAR) {
-      if (!(r2 > hc.near2)) hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rq);
-    }
     const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
     const float dn5 = fmaf(dx, nx5, fmaf(dy, ny5, dz * nz5));
     const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
-    const float e = fmaf(Rq, dm * dn5, Pc * mn);
+    float e;
+    if constexpr (NEAR) {
+      float Rq = Pc * ir2;
+      if (!(r2 > hc.near2)) hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rq);
+      e = fmaf(Rq, dm * dn5, Pc * mn);
+    } else {
+      e = Pc * fmaf(dm * ir2, dn5, mn);  // Rq = Pc ir^2 factored out
+    }
     const float f5 = Pc * dn5;
     dx0 = fmaf(dx, e, dx0);
     dy0 = fmaf(dy, e, dy0);
@@ -2116,7 +2119,7 @@ struct FmmTables {
 // context and h: the field's (make_field_table) and the force's
 // (make_hess_table).
 struct FieldTable {
-  DeviceBuffer buf;  // kFieldTabEntries or kHessTabEntries x 8 floats
+  DeviceBuffer buf;  // kFieldTabEntries or kHessTabEntries x 8 floats, split layout
 };
 
 struct FmmState {
@@ -2353,9 +2356,11 @@ Status build_field_table(mf_ctx *ctx, double h, bool hess, std::shared_ptr<Field
     return Status::Ok();
   }
   auto t = std::make_shared<FieldTable>();
-  std::vector<float> tab(8 * (hess ? kHessTabEntries : kFieldTabEntries));
-  if (hess) make_hess_table(h, tab.data());
-  else make_field_table(h, tab.data());
+  const int entries = hess ? kHessTabEntries : kFieldTabEntries;
+  std::vector<float> rows(8 * entries), tab(8 * entries);
+  if (hess) make_hess_table(h, rows.data());
+  else make_field_table(h, rows.data());
+  split_near_table(rows.data(), entries, tab.data());
   MF_TRY(t->buf.reserve(tab.size() * sizeof(float)));
   MF_HIP(hipMemcpy(t->buf.get(), tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice));
   ctx->cache[key] = t;

@@ -58,6 +58,10 @@ struct FieldConsts {
 // clamp.  At FP32 inputs q in [0, 2) the error is at most 1.5e-7 of
 // max(|alpha|, (2h)^-5) and of max(|beta|, (2h)^-3); the FP32 closed form of
 // field_near_coeffs stays within 7e-7 and 1.6e-6.
+//
+// Device copies of both tables (split_near_table) hold the first curve's
+// cubic of every entry, then the second's: at a 16-byte stride, lanes that
+// read neighbouring entries meet in half as many LDS banks as at 32 bytes.
 constexpr int kFieldTabN = 64;
 constexpr int kFieldTabEntries = kFieldTabN + 1;
 
@@ -91,6 +95,8 @@ FieldConsts make_field_consts(double h);
 HessConsts make_hess_consts(double h);
 void make_field_table(double h, float *tab);  // kFieldTabEntries x 8 floats
 void make_hess_table(double h, float *tab);   // kHessTabEntries x 8 floats
+// [entries][8] -> [2][entries][4]: the device layout of both tables.
+void split_near_table(const float *tab, int entries, float *out);
 
 // ---------------------------------------------------------------------------
 // Field pair coefficients (FP32).  Returns alpha, beta such that the pair adds
@@ -123,7 +129,7 @@ __device__ __forceinline__ void field_near_coeffs(const FieldConsts &fc, float r
   beta = inner ? b_in : b_out;
 }
 
-// The same coefficients from the table (kFieldTabN x 2 float4, in LDS), for
+// The same coefficients from the table (split layout, in LDS), for
 // r2 < (2h)^2: 12 VALU operations and two LDS reads instead of about 30.
 __device__ __forceinline__ void field_near_coeffs_tab(const float4 *tab, float tab_scale, float r2,
                                                       float ir, float &alpha, float &beta) {
@@ -134,7 +140,7 @@ __device__ __forceinline__ void field_near_coeffs_tab(const float4 *tab, float t
   const float u = r2 * tab_scale * irc;
   const int k = static_cast<int>(u);
   const float t = __builtin_amdgcn_fractf(u);
-  const float4 a = tab[2 * k], b = tab[2 * k + 1];
+  const float4 a = tab[k], b = tab[kFieldTabEntries + k];
   alpha = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
   beta = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
 }
@@ -169,7 +175,7 @@ __device__ __forceinline__ void hess_near_coeffs_tab(const float4 *tab, float ta
   const float kc = ceilf(u);              // entry 0 .. kHessTabN + 1
   const float t = u - kc;
   const int e = static_cast<int>(kc);
-  const float4 a = tab[2 * e], b = tab[2 * e + 1];
+  const float4 a = tab[e], b = tab[kHessTabEntries + e];
   const float c1 = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
   const float dq = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
   P = c1 * irc;
