@@ -1630,18 +1630,24 @@ __device__ __forceinline__ void force_range(const SrcRec *__restrict__ rec, int 
     const float dx = tx - sx, dy = ty - sy, dz = tz - sz;
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     const float ir = __builtin_amdgcn_rsqf(r2);
-    const float ir2 = ir * ir;
-    float Pc = ir2 * ir2 * ir;
     const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
     const float dn5 = fmaf(dx, nx5, fmaf(dy, ny5, dz * nz5));
     const float mn = fmaf(mx, nx, fmaf(my, ny, mz * nz));
-    float e;
+    float Pc, e;
     if constexpr (NEAR) {
-      float Rq = Pc * ir2;
-      if (!(r2 > hc.near2)) hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rq);
-      e = fmaf(Rq, dm * dn5, Pc * mn);
+      // Both curves default to their far form r^-4 (P = r^-5, R / -5 = r^-7)
+      // and the table replaces them within 4h; one scaling serves both.
+      // The clamp keeps a pair at d = 0 finite (its terms vanish with d).
+      const float irc = __builtin_amdgcn_fmed3f(ir, 0.f, 1e12f);
+      const float ir2 = irc * irc;
+      float c1 = ir2 * ir2, dq = c1;
+      if (!(r2 > hc.near2)) hess_near_curves_tab(tab, hc.tab_scale, r2, irc, c1, dq);
+      Pc = c1 * irc;
+      e = fmaf(dq * (ir2 * irc), dm * dn5, Pc * mn);
     } else {
-      e = Pc * fmaf(dm * ir2, dn5, mn);  // Rq = Pc ir^2 factored out
+      const float ir2 = ir * ir;
+      Pc = ir2 * ir2 * ir;
+      e = Pc * fmaf(dm * ir2, dn5, mn);  // R / -5 = P ir^2 factored out
     }
     const float f5 = Pc * dn5;
     dx0 = fmaf(dx, e, dx0);
@@ -1925,13 +1931,13 @@ __device__ __forceinline__ void near_force_remainder(const SrcRec *__restrict__ 
             const float dx = tx[t] - pa.x, dy = ty[t] - pa.y, dz = tz[t] - pa.z;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             const float ir = __builtin_amdgcn_rsqf(r2);
-            const float ir2 = ir * ir;
-            float Pc = ir2 * ir2 * ir;
-            float Rc = -5.f * Pc * ir2;
-            if (mixed && !(r2 > hc.near2)) {
-              hess_near_coeffs_tab(tab, hc.tab_scale, r2, ir, Pc, Rc);
-              Rc *= -5.f;
-            }
+            // far curves r^-4 by default, the table's within 4h (force_range)
+            const float irc = __builtin_amdgcn_fmed3f(ir, 0.f, 1e12f);
+            const float ir2 = irc * irc;
+            float c1 = ir2 * ir2, dq = c1;
+            if (mixed && !(r2 > hc.near2)) hess_near_curves_tab(tab, hc.tab_scale, r2, irc, c1, dq);
+            const float Pc = c1 * irc;
+            const float Rc = -5.f * dq * (ir2 * irc);
             const float dm = fmaf(dx, mx, fmaf(dy, my, dz * mz));
             const float dn = fmaf(dx, nx[t], fmaf(dy, ny[t], dz * nz[t]));
             const float mn = fmaf(mx, nx[t], fmaf(my, ny[t], mz * nz[t]));

@@ -164,22 +164,20 @@ __device__ __forceinline__ void hess_near_coeffs(const float *tab, float inv_h, 
   R = live ? dd * (ir * ir * ir) : 0.f;
 }
 
-// P and R / -5 of a force pair within 4h (r2 <= near2) from the table (LDS).
-// The inverse distance is clamped to a finite value, so a pair at d = 0 (a
-// particle and itself, or two coincident ones) gets finite coefficients,
-// whose terms all vanish with d.
-__device__ __forceinline__ void hess_near_coeffs_tab(const float4 *tab, float tab_scale, float r2,
-                                                     float ir, float &P, float &Rq) {
-  const float irc = __builtin_amdgcn_fmed3f(ir, 0.f, 1e12f);
+// The two curves of a force pair within 4h (r2 <= near2) from the table
+// (LDS): P = c1 irc and R / -5 = dq irc^3.  The caller clamps the inverse
+// distance irc to a finite value, so a pair at d = 0 (a particle and itself,
+// or two coincident ones) gets finite coefficients, whose terms all vanish
+// with d.
+__device__ __forceinline__ void hess_near_curves_tab(const float4 *tab, float tab_scale, float r2,
+                                                     float irc, float &c1, float &dq) {
   const float u = r2 * irc * tab_scale;  // in [0, kHessTabN], up to rounding
   const float kc = ceilf(u);              // entry 0 .. kHessTabN + 1
   const float t = u - kc;
   const int e = static_cast<int>(kc);
   const float4 a = tab[e], b = tab[kHessTabEntries + e];
-  const float c1 = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
-  const float dq = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
-  P = c1 * irc;
-  Rq = dq * (irc * irc * irc);
+  c1 = fmaf(fmaf(fmaf(a.w, t, a.z), t, a.y), t, a.x);
+  dq = fmaf(fmaf(fmaf(b.w, t, b.z), t, b.y), t, b.x);
 }
 
 }  // namespace mf

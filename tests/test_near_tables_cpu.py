@@ -4,7 +4,7 @@ make_hess_table in magnetic-force_amd/csrc/mf_allpairs.hip).
 Force: against the oracle's fitted curves (oracle/mf_oracle.c mfo_curves,
 the reference's Dipole_hessian.kern:53-61 quartic pieces on (0, 1], (1, 2],
 (2, 3], (3, 4]).  The lookup below restates the device function
-hess_near_coeffs_tab (mf_pair.hpp): u = q kHessTabN / 4, entry e = ceil(u),
+hess_near_curves_tab (mf_pair.hpp): u = q kHessTabN / 4, entry e = ceil(u),
 t = u - e, two cubics in t.  The pieces jump at integer q (C1 by 7e-4 of its
 value at q = 1), so a table that put an integer q into the upper piece, or an
 interval that straddled two pieces, would miss by far more than the bound.
