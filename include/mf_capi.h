@@ -60,8 +60,8 @@ typedef enum mf_m2l_precision {
                         translation (inner corrections of the refined solve) */
   MF_M2L_BF16X1_CUT = 4 /* MF_M2L_BF16X1 with the translation cut to the
                         16 x 32 blocks whose lowest local and multipole
-                        degrees add up to at most the expansion order (13 of 32
-                        blocks at order 10): ~1e-4 relative per product, for the
+                        degrees add up to at most min(order, 7) (5 of 32
+                        blocks at order 10): ~2e-4 relative per product, for the
                         late products of a relaxed solve (relax_m2l_precision).
                         M2M and L2L run as MF_M2L_BF16X1. */
 } mf_m2l_precision;

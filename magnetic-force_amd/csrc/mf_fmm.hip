@@ -848,8 +848,11 @@ struct DegreeBlocks {
   // blocks that hold real dofs (the others are zero padding, skipped)
   static constexpr bool real(int rb, int ks) { return 16 * rb < ND && 32 * ks < ND; }
   static constexpr int min_sum(int rb, int ks) { return dof_degree(16 * rb) + dof_degree(32 * ks); }
-  // kept by the cut: lowest degrees add up to at most P
-  static constexpr bool low(int rb, int ks) { return real(rb, ks) && min_sum(rb, ks) <= P; }
+  // kept by the cut: lowest degrees add up to at most min(P, 7).  The cut
+  // serves the relaxed products, made once the residual is small, whose error
+  // budget admits a translation of order 7 (5 of 32 blocks at P = 10)
+  static constexpr int kCutSum = P < 7 ? P : 7;
+  static constexpr bool low(int rb, int ks) { return real(rb, ks) && min_sum(rb, ks) <= kCutSum; }
   // three products in the split: lowest degrees add up to at most P - 3
   static constexpr bool low3(int rb, int ks) { return real(rb, ks) && min_sum(rb, ks) <= P - 3; }
   // MFMA products of row block rb per column block
@@ -865,7 +868,7 @@ struct DegreeBlocks {
 // and the operator fragments it loads.  Row blocks go to waves longest
 // first, each to the least loaded wave (P = 10, KP = 128: 12, 10, 10, 10
 // products per column block with SPLIT, against 24 per wave for m2l_kernel;
-// 4, 3, 3, 3 for the cut, against 8).
+// 2, 1, 1, 1 for the cut, against 8).
 struct DegWave {
   int nr;  // row blocks owned (all written; zeros where nothing is multiplied)
   int rb[8];

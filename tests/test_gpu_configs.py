@@ -124,8 +124,8 @@ def test_pool8m_shape_bf16x3_force_vs_fp64_solution(ctx):
     print(f"pool8m-shape: x_err={ex:.3e} F_err={ef:.3e} iters={res.stats['iterations']}")
     assert ex < 5e-6
     assert ef < 1e-5
-    # the bench's setting: products made below a residual of 1e-2 on the
-    # degree-cut one-part translations
+    # relaxed products: those made below a residual of 1e-2 (one more than the
+    # bench's 5e-3 relaxes on the pool) on the degree-cut one-part translations
     rel = plan.solve(kappa, b, rtol=1e-6, relax_m2l_precision=magforce.M2L_BF16X1_CUT,
                      relax_rtol=1e-2)
     assert rel.stats["converged"] == 1 and rel.stats["relaxed_matvecs"] >= 1

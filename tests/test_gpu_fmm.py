@@ -86,7 +86,7 @@ def test_fmm_bf16_m2l_matches_oracle(ctx, slab_case, order, prec):
 def test_fmm_bf16x1_cut_matches_oracle(ctx, slab_case, order):
     """Degree-cut one-part translations (the relaxed products' M2L): the cut
     drops the MFMA blocks whose every term has local plus multipole degree
-    above the order, terms that fall off like rho^(j + n).  Field and force
+    above min(order, 7), terms that fall off like rho^(j + n).  Field and force
     stay within a few 1e-4 of the oracle and near the uncut one-part product;
     a wrongly dropped low-degree block would put them off by 1e-2 or more,
     and a cut that dropped nothing would leave them equal to it."""
