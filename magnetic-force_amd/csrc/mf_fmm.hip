@@ -2445,7 +2445,7 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
   if (nsys == 1) off = {0, n};
   std::vector<int64_t> ch;
   std::vector<int> ch_sys;
-  constexpr int64_t kChunk = 65536;
+  constexpr int64_t kChunk = 4096;  // 282 workgroups on 1.15 M particles
   for (int sy = 0; sy < nsys; ++sy)
     for (int64_t b0 = off[sy]; b0 < off[sy + 1]; b0 += kChunk) {
       ch.push_back(b0);
