@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -606,6 +607,14 @@ constexpr int m2l_min_waves() {
   return (NPART == 3 || (NPART == 2 && KP != 128)) ? 1 : 3;
 }
 
+// Source slots of the tiles from first_tile on, when one launch covers tiles
+// whose slot arrays differ (the force M2L: the leaf level reads the force
+// lists, the levels above it the field's).  first_tile = INT_MAX: none.
+struct M2LSrc2 {
+  const int32_t *src;
+  int ld, empty, first_tile;
+};
+
 template <int KP, int NPART, bool OCTANT = false, bool FENCE = false>
 __global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(m2l_min_waves<KP, NPART>())))
 void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
@@ -622,7 +631,7 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
                                                           const int32_t *__restrict__ tile_list,
                                                           const int32_t *__restrict__ tile_nlist,
                                                           int list_ld, int accumulate,
-                                                          SysMask mask) {
+                                                          SysMask mask, M2LSrc2 src2) {
   constexpr int NT = 2 * KP;
   constexpr int LDF = KP + 4;  // FP32 row pitch (floats)
   // BF16 row pitch (elements).  Pitches of KP + 8, + 16 and + 24 are all
@@ -636,6 +645,11 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
   const int per_xcd = (ntiles + 7) / 8;
   const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
   if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  if (tile_id >= src2.first_tile) {
+    src_slots = src2.src;
+    src_ld = src2.ld;
+    empty_idx = src2.empty;
+  }
   const M2LTile tile = tiles[tile_id];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -1065,13 +1079,18 @@ void m2l_deg_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *_
                     const int32_t *__restrict__ sorted_boxes, const int32_t *__restrict__ src_slots,
                     int src_ld, int empty_idx, const int32_t *__restrict__ slot_table, int slot_ld,
                     const int32_t *__restrict__ tile_list, const int32_t *__restrict__ tile_nlist,
-                    int list_ld, SysMask mask) {
+                    int list_ld, SysMask mask, M2LSrc2 src2) {
   constexpr int NT = 2 * KP, LDB = SPLIT ? KP + 24 : KP + 8;
   __shared__ __attribute__((aligned(16))) uint16_t sB[(SPLIT ? 2 : 1) * kTileT * LDB];
   __shared__ int s_tgt[kTileT];
   const int per_xcd = (ntiles + 7) / 8;
   const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
   if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  if (tile_id >= src2.first_tile) {
+    src_slots = src2.src;
+    src_ld = src2.ld;
+    empty_idx = src2.empty;
+  }
   const M2LTile tile = tiles[tile_id];
   const int tid = threadIdx.x;
   for (int j = tid; j < kTileT; j += NT) s_tgt[j] = j < tile.count ? sorted_boxes[tile.first + j] : -1;
@@ -2979,7 +2998,8 @@ namespace {
 template <int P, int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
                   int ntiles, const int32_t *src, int src_rows, int nslot, const int32_t *tlist,
-                  const int32_t *tn, SysMask mask, bool fence) {
+                  const int32_t *tn, SysMask mask, bool fence,
+                  M2LSrc2 src2 = M2LSrc2{nullptr, 0, 0, INT_MAX}) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
   const int npart = prec == MF_M2L_FP32 ? 0 : 3;
@@ -2990,11 +3010,11 @@ Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, cons
   ctx->stream, msrc, msrc_bytes, F.Lt.as<float>(), tables, tiles, ntiles,                     \
       F.class_ids2.as<int32_t>(), src, nslot, src_rows, tab.slot_tab.as<int32_t>(),            \
       tab.slot_ld, tlist,                                                                      \
-      tn, tab.slot_ld, 0, mask
+      tn, tab.slot_ld, 0, mask, src2
 #define MF_DEG_LAUNCH                                                                         \
   ctx->stream, F.Mb.get(), msrc_bytes, F.Lt.as<float>(), tab.Tb.as<uint16_t>(), tiles, ntiles, \
       F.class_ids2.as<int32_t>(), src, nslot, src_rows, tab.slot_tab.as<int32_t>(),            \
-      tab.slot_ld, tlist, tn, tab.slot_ld, mask
+      tab.slot_ld, tlist, tn, tab.slot_ld, mask, src2
   switch (prec) {
     case MF_M2L_FP32: {
       const void *tables = tab.T.get();
@@ -3083,7 +3103,8 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &l
         static_cast<uint32_t>(static_cast<int64_t>(F.nbox) * (npart ? 6 * KP : 4 * KP));
 #define MF_OCTANT_ARGS                                                                          \
   ctx->stream, msrc, msrc_bytes, out, tables, tiles, nt, boxes, src, src_ld, F.nbox,          \
-      tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask
+      tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask,       \
+      M2LSrc2{nullptr, 0, 0, INT_MAX}
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2, true>), grid, block, 0, MF_OCTANT_ARGS);
@@ -3124,15 +3145,13 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &l
                             false)));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
-      // force lists srcH (indexed by leaf = global index), the others srcF.
+      // force lists srcH (indexed by leaf = global index), the others srcF;
+      // one launch, so the upper levels' few tiles run beside the leaves'
       const int lt = F.leaf_tile_start;
-      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), lt, F.srcF.as<int32_t>(),
-                            F.nbox, tab.nslotF, F.tlistH.as<int32_t>(), F.tnH.as<int32_t>(), mask,
-                            true)));
-      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt,
-                            F.srcH.as<int32_t>(), F.count[D], tab.nslotH,
-                            F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * tab.slot_ld,
-                            F.tnH.as<int32_t>() + lt, mask, true)));
+      MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), F.ntilesH,
+                            F.srcF.as<int32_t>(), F.nbox, tab.nslotF, F.tlistH.as<int32_t>(),
+                            F.tnH.as<int32_t>(), mask, true,
+                            M2LSrc2{F.srcH.as<int32_t>(), tab.nslotH, F.count[D], lt})));
     }
   }
   {
