@@ -613,7 +613,27 @@ constexpr int m2l_min_waves() {
 struct M2LSrc2 {
   const int32_t *src;
   int ld, empty, first_tile;
+  const int32_t *order;  // tile order within each XCD's range, or null
 };
+
+// Tile order of an M2L launch: each XCD takes a contiguous range of tiles
+// (per = ceil(ntiles / 8)); within it, the tiles with the most non-empty
+// offsets go first, so the launch does not end on a long tile started last.
+// order[r0 + rank] = t, ranked by descending count, ties by index.
+__global__ void tile_order_kernel(int ntiles, const int32_t *__restrict__ tn,
+                                  int32_t *__restrict__ order) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles) return;
+  const int per = (ntiles + 7) / 8;
+  const int r0 = (t / per) * per, r1 = min(r0 + per, ntiles);
+  const int nt = tn[t];
+  int rank = 0;
+  for (int u = r0; u < r1; ++u) {
+    const int nu = tn[u];
+    rank += (nu > nt) || (nu == nt && u < t);
+  }
+  order[r0 + rank] = t;
+}
 
 template <int KP, int NPART, bool OCTANT = false, bool FENCE = false>
 __global__ __launch_bounds__(2 * KP) __attribute__((amdgpu_waves_per_eu(m2l_min_waves<KP, NPART>())))
@@ -643,8 +663,9 @@ void m2l_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes,
   __shared__ __attribute__((aligned(16))) unsigned char s_raw[kLds];
   __shared__ int s_tgt[kTileT];
   const int per_xcd = (ntiles + 7) / 8;
-  const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+  int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
   if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  if (src2.order) tile_id = src2.order[tile_id];
   if (tile_id >= src2.first_tile) {
     src_slots = src2.src;
     src_ld = src2.ld;
@@ -1084,8 +1105,9 @@ void m2l_deg_kernel(const void *__restrict__ Msrc, uint32_t msrc_bytes, float *_
   __shared__ __attribute__((aligned(16))) uint16_t sB[(SPLIT ? 2 : 1) * kTileT * LDB];
   __shared__ int s_tgt[kTileT];
   const int per_xcd = (ntiles + 7) / 8;
-  const int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
+  int tile_id = static_cast<int>(blockIdx.x % 8) * per_xcd + static_cast<int>(blockIdx.x / 8);
   if (tile_id >= ntiles) return;  // uniform over the workgroup, before any barrier
+  if (src2.order) tile_id = src2.order[tile_id];
   if (tile_id >= src2.first_tile) {
     src_slots = src2.src;
     src_ld = src2.ld;
@@ -2207,7 +2229,8 @@ struct FmmState {
   DeviceBuffer srcF, srcH;   // M2L source slots [nbox][nslotF], [nleaf][nslotH]
   DeviceBuffer class_ids, class_keys, class_keys2, class_ids2, class_counts;
   DeviceBuffer tilesF, tilesH;
-  DeviceBuffer tlistF, tlistH, tnF, tnH;  // non-empty offsets per tile [ntiles][slot_ld], counts
+  DeviceBuffer tlistF, tlistH, tnF, tnH;
+  DeviceBuffer orderF, orderH;  // launch order of the tiles (tile_order_kernel)  // non-empty offsets per tile [ntiles][slot_ld], counts
   int ntilesF = 0, ntilesH = 0;
   int leaf_tile_start = 0;  // first tile of the leaf level (tiles are in level order)
   // Upward and downward translations on the M2L kernel: M2M tiles (parents,
@@ -2799,6 +2822,18 @@ Status fmm_build_at(mf_plan *plan, double s_leaf) {
     MF_TRY(tile_lists(F.tilesH.as<M2LTile>() + lt, F.ntilesH - lt, F.srcH.as<int32_t>(), tab.nslotH,
                       F.tlistH.as<int32_t>() + static_cast<int64_t>(lt) * ld,
                       F.tnH.as<int32_t>() + lt));
+    MF_TRY(need<int32_t>(F.orderF, std::max(F.ntilesF, 1)));
+    MF_TRY(need<int32_t>(F.orderH, std::max(F.ntilesH, 1)));
+    if (F.ntilesF > 0) {
+      hipLaunchKernelGGL(tile_order_kernel, dim3(grid_for(F.ntilesF, 256)), dim3(256), 0, ctx->stream,
+                         F.ntilesF, F.tnF.as<int32_t>(), F.orderF.as<int32_t>());
+      MF_LAUNCH_CHECK();
+    }
+    if (F.ntilesH > 0) {
+      hipLaunchKernelGGL(tile_order_kernel, dim3(grid_for(F.ntilesH, 256)), dim3(256), 0, ctx->stream,
+                         F.ntilesH, F.tnH.as<int32_t>(), F.orderH.as<int32_t>());
+      MF_LAUNCH_CHECK();
+    }
     // M2M / L2L tiles
     MF_TRY(need<int32_t>(F.child_tab, static_cast<int64_t>(F.nbox + 1) * 8));
     MF_TRY(need<int32_t>(F.box_ids, F.nbox));
@@ -2999,7 +3034,7 @@ template <int P, int KP>
 Status launch_m2l(mf_ctx *ctx, FmmState &F, const FmmTables &tab, int prec, const M2LTile *tiles,
                   int ntiles, const int32_t *src, int src_rows, int nslot, const int32_t *tlist,
                   const int32_t *tn, SysMask mask, bool fence,
-                  M2LSrc2 src2 = M2LSrc2{nullptr, 0, 0, INT_MAX}) {
+                  M2LSrc2 src2 = M2LSrc2{nullptr, 0, 0, INT_MAX, nullptr}) {
   if (ntiles <= 0) return Status::Ok();
   const dim3 grid(8 * ((ntiles + 7) / 8)), block(2 * KP);
   const int npart = prec == MF_M2L_FP32 ? 0 : 3;
@@ -3104,7 +3139,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &l
 #define MF_OCTANT_ARGS                                                                          \
   ctx->stream, msrc, msrc_bytes, out, tables, tiles, nt, boxes, src, src_ld, F.nbox,          \
       tab.slot_tab.as<int32_t>(), tab.slot_ld, tlist, tn, tab.slot_ld, accumulate, mask,       \
-      M2LSrc2{nullptr, 0, 0, INT_MAX}
+      M2LSrc2{nullptr, 0, 0, INT_MAX, nullptr}
     if (npart == 0) hipLaunchKernelGGL((m2l_kernel<KP, 0, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 1) hipLaunchKernelGGL((m2l_kernel<KP, 1, true>), grid, block, 0, MF_OCTANT_ARGS);
     else if (npart == 2) hipLaunchKernelGGL((m2l_kernel<KP, 2, true>), grid, block, 0, MF_OCTANT_ARGS);
@@ -3142,7 +3177,7 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &l
     if (!hess) {
       MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesF.as<M2LTile>(), F.ntilesF, F.srcF.as<int32_t>(),
                             F.nbox, tab.nslotF, F.tlistF.as<int32_t>(), F.tnF.as<int32_t>(), mask,
-                            false)));
+                            false, M2LSrc2{nullptr, 0, 0, INT_MAX, F.orderF.as<int32_t>()})));
     } else {
       // Tiles are in level order: the leaf level's (trailing) tiles read the
       // force lists srcH (indexed by leaf = global index), the others srcF;
@@ -3151,7 +3186,8 @@ Status upward_and_m2l(mf_plan *plan, bool hess, const std::function<Status()> &l
       MF_TRY((launch_m2l<P, KP>(ctx, F, tab, prec, F.tilesH.as<M2LTile>(), F.ntilesH,
                             F.srcF.as<int32_t>(), F.nbox, tab.nslotF, F.tlistH.as<int32_t>(),
                             F.tnH.as<int32_t>(), mask, true,
-                            M2LSrc2{F.srcH.as<int32_t>(), tab.nslotH, F.count[D], lt})));
+                            M2LSrc2{F.srcH.as<int32_t>(), tab.nslotH, F.count[D], lt,
+                                    F.orderH.as<int32_t>()})));
     }
   }
   {
