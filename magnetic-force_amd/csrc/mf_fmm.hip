@@ -607,9 +607,11 @@ constexpr int m2l_min_waves() {
   return (NPART == 3 || (NPART == 2 && KP != 128)) ? 1 : 3;
 }
 
-// Source slots of the tiles from first_tile on, when one launch covers tiles
-// whose slot arrays differ (the force M2L: the leaf level reads the force
-// lists, the levels above it the field's).  first_tile = INT_MAX: none.
+// Per-launch tile options of the M2L kernels.  src, ld, empty: the source
+// slots of the tiles from first_tile on, when one launch covers tiles whose
+// slot arrays differ (the force M2L: the leaf level reads the force lists,
+// the levels above it the field's); first_tile = INT_MAX: none.  order: the
+// launch order of the tiles (tile_order_kernel), or null for index order.
 struct M2LSrc2 {
   const int32_t *src;
   int ld, empty, first_tile;
