@@ -539,9 +539,10 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t (&p)[NPART]) {
 // [box][3][KP]), of which the BF16 gather may take the first KG columns of
 // each part only.  Chunk e (e = tid + it * 2 KP) is target e / PER_T, chunk
 // e % PER_T of that target.
-template <int KP, int NPART, int KG = KP>
+template <int KP, int NPART, int KG = KP, int KG1 = KG>
 struct M2LGather {
-  static constexpr int PER_T = NPART == 0 ? KP / 4 : NPART * KG / 8;  // chunks per target
+  // chunks per target: KG columns of part 0, KG1 of each further part
+  static constexpr int PER_T = NPART == 0 ? KP / 4 : (KG + (NPART - 1) * KG1) / 8;
   static constexpr int IT = 64 * PER_T / (2 * KP);                    // chunks per thread
   static_assert(IT * 2 * KP == 64 * PER_T, "the tile's chunks split evenly over the threads");
 };
@@ -552,11 +553,12 @@ struct M2LGather {
 // under a branch makes the compiler wait for all outstanding loads (these
 // and the operator prefetch) at the first use of any of them, which would
 // serialize the gather with the MFMAs.
-template <int KP, int NPART, int KG = KP>
+template <int KP, int NPART, int KG = KP, int KG1 = KG>
 __device__ __forceinline__ void m2l_mult_load(__amdgpu_buffer_rsrc_t msrc,
-                                              const int (&src)[M2LGather<KP, NPART, KG>::IT], int tid,
-                                              float4 (&v)[M2LGather<KP, NPART, KG>::IT]) {
-  using G = M2LGather<KP, NPART, KG>;
+                                              const int (&src)[M2LGather<KP, NPART, KG, KG1>::IT],
+                                              int tid,
+                                              float4 (&v)[M2LGather<KP, NPART, KG, KG1>::IT]) {
+  using G = M2LGather<KP, NPART, KG, KG1>;
   constexpr uint32_t kBoxBytes = NPART == 0 ? KP * 4 : 3 * KP * 2;
   typedef uint32_t u4 __attribute__((ext_vector_type(4)));
 #pragma unroll
@@ -564,8 +566,12 @@ __device__ __forceinline__ void m2l_mult_load(__amdgpu_buffer_rsrc_t msrc,
     const uint32_t r = static_cast<uint32_t>((tid + it * 2 * KP) % G::PER_T);
     // BF16: chunk r of the box's [3][KP] parts block; src = -1 wraps to
     // 2^32 - kBoxBytes, beyond any range the host accepts
-    // chunk r: part r / (KG / 8), columns 8 (r % (KG / 8)) .. + 8 of it
-    const uint32_t cpart = NPART == 0 ? 0u : r / (KG / 8), cq = NPART == 0 ? r : r % (KG / 8);
+    // chunk r: part 0 for r < KG / 8, then KG1 / 8 chunks of each further part
+    uint32_t cpart = 0, cq = r;
+    if (NPART != 0 && r >= KG / 8) {
+      cpart = 1 + (r - KG / 8) / (KG1 / 8);
+      cq = (r - KG / 8) % (KG1 / 8);
+    }
     const uint32_t off = static_cast<uint32_t>(src[it]) * kBoxBytes + cpart * (2u * KP) + 16u * cq;
     const u4 x = __builtin_amdgcn_raw_buffer_load_b128(msrc, static_cast<int>(off), 0, 0);
     v[it] = make_float4(__uint_as_float(x.x), __uint_as_float(x.y), __uint_as_float(x.z),
@@ -970,6 +976,20 @@ constexpr DegWave deg_wave(int w) {
 // whole while the current offset multiplies.
 // Multipole columns the degree kernel reads: through the last k step that any
 // wave multiplies (the cut at P = 10 reads dofs 0 .. 63 only).
+// Part-1 columns of the split (the three-product blocks' k steps).
+template <int KP, int P, bool SPLIT>
+constexpr int deg_gather_width1() {
+  using C = DegreeBlocks<KP, P>;
+  if (!SPLIT) return KP;
+  int last = 0;
+  for (int ks = 0; ks < C::NK; ++ks)
+    for (int rb = 0; rb < C::NRB; ++rb)
+      if (C::low3(rb, ks)) last = ks;
+  const int kg1 = 32 * (last + 1);
+  // the whole width where the chunks would not split evenly over the threads
+  return (64 * ((KP + kg1) / 8)) % (2 * KP) == 0 ? kg1 : KP;
+}
+
 template <int KP, int P, bool SPLIT>
 constexpr int deg_gather_width() {
   using C = DegreeBlocks<KP, P>;
@@ -991,9 +1011,10 @@ __device__ __forceinline__ void m2l_deg_wave(
   constexpr DegWave S = deg_wave<KP, P, SPLIT>(W);
   constexpr int NPG = SPLIT ? 2 : 1;  // multipole parts gathered
   constexpr int KG = deg_gather_width<KP, P, SPLIT>();  // multipole columns gathered
+  constexpr int KG1 = deg_gather_width1<KP, P, SPLIT>();  // of part 1 (split)
   constexpr int NT = 2 * KP, LDB = SPLIT ? KP + 24 : KP + 8;
   constexpr int NR = S.nr > 0 ? S.nr : 1, NF = S.nf > 0 ? S.nf : 1;
-  using GA = M2LGather<KP, NPG, KG>;
+  using GA = M2LGather<KP, NPG, KG, KG1>;
   typedef float f4 __attribute__((ext_vector_type(4)));
   const int lane = tid & 63, r16 = lane & 15, kq = lane >> 4;
   f4 acc[NR][4];
@@ -1031,7 +1052,7 @@ __device__ __forceinline__ void m2l_deg_wave(
   bf16x8 fa[NF], fb[NF];
   if (na > 0) {
     src_load(tl[0], true);
-    m2l_mult_load<KP, NPG, KG>(msrc_rsrc, src, tid, v);
+    m2l_mult_load<KP, NPG, KG, KG1>(msrc_rsrc, src, tid, v);
     src_load(tl[min(1, na - 1)], na > 1);
     a_load(table_of(0), fa);
   }
@@ -1041,11 +1062,12 @@ __device__ __forceinline__ void m2l_deg_wave(
     for (int it = 0; it < GA::IT; ++it) {
       const int e = tid + it * NT;
       const int j = e / GA::PER_T, q = e % GA::PER_T;
-      const int p = q / (KG / 8), q8 = q % (KG / 8);
+      const int p = q < KG / 8 ? 0 : 1 + (q - KG / 8) / (KG1 / 8);
+      const int q8 = q < KG / 8 ? q : (q - KG / 8) % (KG1 / 8);
       *reinterpret_cast<float4 *>(&sB[(p * kTileT + j) * LDB + 8 * q8]) = v[it];
     }
     __syncthreads();
-    m2l_mult_load<KP, NPG, KG>(msrc_rsrc, src, tid, v);
+    m2l_mult_load<KP, NPG, KG, KG1>(msrc_rsrc, src, tid, v);
     src_load(tl[min(pc + 2, na - 1)], pc + 2 < na);
     a_load(table_of(min(pc + 1, na - 1)), nxt);
     // the scheduler would otherwise move these prefetches next to their
